@@ -1,0 +1,150 @@
+/*
+ * sdfx.h — C ABI of libsdfx, the MI355X (gfx950) implementation of
+ * mesh_to_sdf's hot path.
+ *
+ * Every entry point below stands in for one function of the reference crate's
+ * public Rust API; the comment above each cites the interface it replaces
+ * (paths relative to mesh_to_sdf/src/ in ZiplineTeam/mesh_to_sdf). The
+ * signatures use plain pointers and sizes only, so a Rust `-sys` crate (see
+ * INTEGRATION.md), ctypes or cgo can bind them without HIP headers.
+ *
+ * Data layout: vertices and query points are packed f32 triples (x, y, z),
+ * i.e. `[[f32; 3]]`; indices are u32. Grid output is f32 per cell with z the
+ * fastest axis (grid.rs:276-278).
+ *
+ * Host-pointer entry points copy their inputs to the device, run, copy the
+ * result back and return when it is in `out` (the reference is synchronous).
+ * `_device` entry points take device pointers and a hipStream_t (passed as
+ * void*), enqueue everything on that stream and synchronise it before
+ * returning the status.
+ *
+ * Errors: the reference panics; here every panic becomes a negative status
+ * and a thread-local message (sdfx_last_error). Conditions are checked in the
+ * order the reference would hit them.
+ *
+ * Thread safety: all entry points are re-entrant. Each calling thread gets its
+ * own stream for host-pointer calls; sdfx_mesh handles are immutable after
+ * build and may be queried from several threads.
+ */
+#ifndef SDFX_H
+#define SDFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDFX_VERSION 1
+
+typedef enum sdfx_status {
+    SDFX_OK = 0,
+    /* Not an error: the reference returns an empty Vec / None here
+     * (generate/generic/rtree_bvh.rs:96-98, 124-126). *out_count is 0. */
+    SDFX_EMPTY_RESULT = 1,
+    /* lib.rs:375 "Query point {..} contains non-finite values" */
+    SDFX_ERR_NON_FINITE_QUERY = -1,
+    /* lib.rs:203 "TriangleList indices length ({}) must be divisible by 3" */
+    SDFX_ERR_INDEX_COUNT = -2,
+    /* lib.rs:207 "TriangleList vertices length ({}) must be divisible by 3 ..." */
+    SDFX_ERR_VERTEX_COUNT = -3,
+    /* slice index out of bounds when a triangle reads vertices[i] */
+    SDFX_ERR_INDEX_OUT_OF_RANGE = -4,
+    /* generic/rtree.rs:148 nearest_neighbor().unwrap() on an empty tree */
+    SDFX_ERR_EMPTY_MESH = -5,
+    /* lib.rs:330 "NaN distance" */
+    SDFX_ERR_NAN_DISTANCE = -6,
+    SDFX_ERR_INVALID_ARGUMENT = -7,
+    SDFX_ERR_DEVICE = -8,
+    SDFX_ERR_OUT_OF_MEMORY = -9,
+    /* grid.rs:311-315 clamp(0, n - 1) panics on a zero cell count */
+    SDFX_ERR_INVALID_GRID = -10
+} sdfx_status;
+
+/* lib.rs:156-171 Topology::{TriangleList, TriangleStrip}; indices == NULL is `None` */
+typedef enum sdfx_topology { SDFX_TRIANGLE_LIST = 0, SDFX_TRIANGLE_STRIP = 1 } sdfx_topology;
+
+/* lib.rs:252-263 SignMethod (Raycast is the default) */
+typedef enum sdfx_sign_method { SDFX_SIGN_RAYCAST = 0, SDFX_SIGN_NORMAL = 1 } sdfx_sign_method;
+
+/* lib.rs:272-286 AccelerationMethod; None and Bvh carry a sign method, Rtree
+ * always uses Normal and RtreeBvh always uses Raycast (the sign argument is
+ * ignored for those two). */
+typedef enum sdfx_acceleration_method {
+    SDFX_ACCEL_NONE = 0,
+    SDFX_ACCEL_BVH = 1,
+    SDFX_ACCEL_RTREE = 2,
+    SDFX_ACCEL_RTREE_BVH = 3
+} sdfx_acceleration_method;
+
+/* Opaque resident-mesh handle: lib.rs:303-312 SdfAccelerationMesh. Owns the
+ * device copy of the mesh and its device LBVH until sdfx_free_sdf_acceleration. */
+typedef struct sdfx_mesh sdfx_mesh;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* sdfx_last_error(void);
+
+/* SDFX_VERSION of the loaded library. */
+int sdfx_version(void);
+
+/* Replaces lib.rs:364-391 generate_sdf<V, I>(vertices, indices, query_points,
+ * acceleration_method) -> Vec<f32>. `out` holds query_count floats;
+ * *out_count receives the number written (query_count, or 0 with
+ * SDFX_EMPTY_RESULT). */
+sdfx_status sdfx_generate_sdf(const float* vertices, uint64_t vertex_count, const uint32_t* indices,
+                              uint64_t index_count, sdfx_topology topology, const float* query_points,
+                              uint64_t query_count, sdfx_acceleration_method method, sdfx_sign_method sign,
+                              float* out, uint64_t* out_count);
+
+/* Device-pointer form of sdfx_generate_sdf (same semantics; `stream` is a
+ * hipStream_t or NULL for the null stream). */
+sdfx_status sdfx_generate_sdf_device(const float* d_vertices, uint64_t vertex_count, const uint32_t* d_indices,
+                                     uint64_t index_count, sdfx_topology topology, const float* d_query_points,
+                                     uint64_t query_count, sdfx_acceleration_method method,
+                                     sdfx_sign_method sign, float* d_out, uint64_t* out_count, void* stream);
+
+/* Replaces generate/grid.rs:265-273 generate_grid_sdf<V, I>(vertices, indices,
+ * grid, sign_method) -> Vec<f32>, with the grid given as Grid::new(first_cell,
+ * cell_size, cell_count) (grid.rs:197-203). `out` holds
+ * cell_count[0]*cell_count[1]*cell_count[2] floats, z fastest. */
+sdfx_status sdfx_generate_grid_sdf(const float* vertices, uint64_t vertex_count, const uint32_t* indices,
+                                   uint64_t index_count, sdfx_topology topology, const float first_cell[3],
+                                   const float cell_size[3], const uint64_t cell_count[3], sdfx_sign_method sign,
+                                   float* out);
+
+/* Device-pointer form of sdfx_generate_grid_sdf. */
+sdfx_status sdfx_generate_grid_sdf_device(const float* d_vertices, uint64_t vertex_count,
+                                          const uint32_t* d_indices, uint64_t index_count, sdfx_topology topology,
+                                          const float first_cell[3], const float cell_size[3],
+                                          const uint64_t cell_count[3], sdfx_sign_method sign, float* d_out,
+                                          void* stream);
+
+/* Replaces lib.rs:410-433 build_sdf_acceleration(vertices, indices,
+ * acceleration_method) -> Option<SdfAccelerationMesh>. *mesh_out is NULL with
+ * SDFX_EMPTY_RESULT where the reference returns None (RtreeBvh on an empty
+ * mesh). */
+sdfx_status sdfx_build_sdf_acceleration(const float* vertices, uint64_t vertex_count, const uint32_t* indices,
+                                        uint64_t index_count, sdfx_topology topology,
+                                        sdfx_acceleration_method method, sdfx_sign_method sign,
+                                        sdfx_mesh** mesh_out);
+
+/* Replaces lib.rs:463-485 query_sdf(&SdfAccelerationMesh, query_points) -> Vec<f32>.
+ * Like the reference, no finiteness check is made on the queries. */
+sdfx_status sdfx_query_sdf(const sdfx_mesh* mesh, const float* query_points, uint64_t query_count, float* out,
+                           uint64_t* out_count);
+
+/* Replaces generic/rtree_bvh.rs:217-276 query_vector_to_closest_point_rtree_bvh
+ * (re-exported at lib.rs:150). `mesh` must have been built with
+ * SDFX_ACCEL_RTREE_BVH. vectors_out holds 3*query_count floats (closest - query),
+ * inside_out query_count bytes (1 = inside by the 3-ray vote). */
+sdfx_status sdfx_query_vector_to_closest_point(const sdfx_mesh* mesh, const float* query_points,
+                                               uint64_t query_count, float* vectors_out, uint8_t* inside_out);
+
+/* Drops an SdfAccelerationMesh (lib.rs:303-312 is Clone + Drop). NULL is allowed. */
+void sdfx_free_sdf_acceleration(sdfx_mesh* mesh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDFX_H */

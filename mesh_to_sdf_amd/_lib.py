@@ -1,0 +1,87 @@
+"""Loader and ctypes signatures for libsdfx.so (the HIP library built from csrc/).
+
+The library is required: if it is missing or fails to load, every entry point
+raises. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsdfx.so")
+CSRC = os.path.join(HERE, "csrc")
+
+OK = 0
+EMPTY_RESULT = 1
+ERR_NON_FINITE_QUERY = -1
+ERR_INDEX_COUNT = -2
+ERR_VERTEX_COUNT = -3
+ERR_INDEX_OUT_OF_RANGE = -4
+ERR_EMPTY_MESH = -5
+ERR_NAN_DISTANCE = -6
+ERR_INVALID_ARGUMENT = -7
+ERR_DEVICE = -8
+ERR_OUT_OF_MEMORY = -9
+ERR_INVALID_GRID = -10
+
+_f = ctypes.POINTER(ctypes.c_float)
+_u32 = ctypes.POINTER(ctypes.c_uint32)
+_u64 = ctypes.POINTER(ctypes.c_uint64)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u64v = ctypes.c_uint64
+
+# name -> (restype, argtypes); must match include/sdfx.h
+SIGNATURES = {
+    "sdfx_last_error": (ctypes.c_char_p, []),
+    "sdfx_version": (_i, []),
+    "sdfx_generate_sdf": (_i, [_f, _u64v, _u32, _u64v, _i, _f, _u64v, _i, _i, _f, _u64]),
+    "sdfx_generate_sdf_device": (_i, [_vp, _u64v, _vp, _u64v, _i, _vp, _u64v, _i, _i, _vp, _u64, _vp]),
+    "sdfx_generate_grid_sdf": (_i, [_f, _u64v, _u32, _u64v, _i, _f, _f, _u64, _i, _f]),
+    "sdfx_generate_grid_sdf_device": (_i, [_vp, _u64v, _vp, _u64v, _i, _f, _f, _u64, _i, _vp, _vp]),
+    "sdfx_build_sdf_acceleration": (_i, [_f, _u64v, _u32, _u64v, _i, _i, _i, ctypes.POINTER(_vp)]),
+    "sdfx_query_sdf": (_i, [_vp, _f, _u64v, _f, _u64]),
+    "sdfx_query_vector_to_closest_point": (_i, [_vp, _f, _u64v, _f, _u8]),
+    "sdfx_free_sdf_acceleration": (None, [_vp]),
+}
+
+_lib = None
+
+
+class SdfxError(RuntimeError):
+    """A status the reference reports by panicking."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+        self.message = message
+
+
+def build(jobs: int = 8) -> str:
+    """Compile libsdfx.so for gfx950 with hipcc (no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", CSRC, f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C {CSRC}")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> int:
+    if status < 0:
+        msg = lib().sdfx_last_error()
+        raise SdfxError(status, msg.decode() if msg else "")
+    return status

@@ -1,0 +1,488 @@
+// sdfx_grid.hip — generate_grid_sdf (generate/grid.rs:265-378) on the device.
+//
+// Cells hold a 64-bit key ordered like the reference's stored distance:
+//   Raycast: bits(d) << 32 | triangle        (d >= 0, so the bits order as floats)
+//   Normal:  bits(|d|) << 32 | negative << 31 | triangle
+// so one atomicMin keeps the smallest distance, then (Normal) the positive
+// one, then the lowest triangle index.
+//
+// 1. Seeding (generate/grid.rs:383-457): every (triangle, cell) pair of each
+//    triangle's snapped, widened padded box, one lane per pair (the pair list
+//    is a prefix sum over the box volumes, so large triangles cannot starve a
+//    lane).
+// 2. Propagation (generate/grid.rs:495-558): the reference's relaxation rule
+//    (a cell's triangle is tried on its neighbours and kept on strict
+//    improvement) run as rounds. Round r snapshots (cell, triangle) of every
+//    cell improved in round r-1, then tries each on its 26 neighbours; the
+//    snapshot makes each round, and so the fixpoint, independent of
+//    execution order.
+// 3. Sign (generate/grid.rs:568-684): the rays start at the cells with index 0
+//    along +X/+Y/+Z. Instead of walking a BVH per ray, every triangle visits
+//    the rays whose origin lies in its padded box (the rays the reference's
+//    BVH traversal would return it for), runs the aligned test and flips bit
+//    k = min(floor(t / size), n - 1) of that ray's row. A suffix XOR along each
+//    row gives the parity of hits at or beyond each cell, i.e. of the
+//    reference's `+1 on cells 0..=k` counters. The 2-of-3 vote is fused into
+//    the output write.
+#include "sdfx_internal.hpp"
+
+namespace sdfx {
+
+constexpr uint64_t KEY_EMPTY = ~0ull;
+
+struct GridDev {
+    float first[3];
+    float size[3];
+    uint32_t n[3];
+    uint32_t words[3];  // u32 words per ray row along each axis
+};
+
+__device__ __forceinline__ float cell_coord(const GridDev& g, int k, uint32_t i) {
+    return g.first[k] + (float)i * g.size[k];  // grid.rs:289-295
+}
+
+__device__ __forceinline__ uint64_t cell_index(const GridDev& g, uint32_t x, uint32_t y, uint32_t z) {
+    return (uint64_t)z + (uint64_t)y * g.n[2] + (uint64_t)x * g.n[1] * g.n[2];  // grid.rs:276-278
+}
+
+// `f32 as isize` (saturating, NaN -> 0) after floor, clamped to [0, n - 1]
+// (grid.rs:299-324).
+__device__ __forceinline__ uint32_t snap_axis(float c, uint32_t n) {
+    float f = floorf(c);
+    int64_t i;
+    if (f != f) i = 0;
+    else if (f >= 9.2233720e18f) i = INT64_MAX;
+    else if (f <= -9.2233720e18f) i = INT64_MIN;
+    else i = (int64_t)f;
+    int64_t hi = (int64_t)n - 1;
+    return (uint32_t)(i < 0 ? 0 : (i > hi ? hi : i));
+}
+
+__device__ __forceinline__ bool key_valid(float d) {
+    float a = fabsf(d);
+    return a == a && fbits(a) < 0x7f7ffffdu;  // compare_distances(d, f32::MAX) is Less
+}
+
+template <int SIGN>
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t tri) {
+    if (SIGN == SDFX_SIGN_RAYCAST) return ((uint64_t)fbits(d) << 32) | tri;
+    return ((uint64_t)fbits(fabsf(d)) << 32) | ((uint64_t)(fbits(d) >> 31) << 31) | tri;
+}
+
+__device__ __forceinline__ uint32_t key_tri(uint64_t k, int sign) {
+    return sign == SDFX_SIGN_NORMAL ? (uint32_t)(k & 0x7fffffffu) : (uint32_t)k;
+}
+
+__device__ __forceinline__ float key_value(uint64_t k, int sign) {
+    if (k == KEY_EMPTY) return FLT_MAX;
+    float a = bitsf((uint32_t)(k >> 32));
+    return (sign == SDFX_SIGN_NORMAL && ((k >> 31) & 1ull)) ? -a : a;
+}
+
+template <int SIGN>
+__device__ __forceinline__ float cell_dist(const GridDev& g, uint32_t x, uint32_t y, uint32_t z, const TriRec& t) {
+    float px = cell_coord(g, 0, x), py = cell_coord(g, 1, y), pz = cell_coord(g, 2, z);
+    return SIGN == SDFX_SIGN_RAYCAST ? tri_dist(px, py, pz, t) : tri_signed_dist(px, py, pz, t);
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u32g(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------- seeding
+
+struct SeedBox {
+    uint32_t mn[3];
+    uint32_t ext[3];  // mx - mn + 1
+};
+
+// generate/grid.rs:400-426: padded box, snapped corners, widened by one cell
+// where the snapped corner's centre lies beyond the box.
+__global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ tris, uint64_t nt, GridDev g,
+                                                    SeedBox* __restrict__ boxes, uint64_t* __restrict__ counts) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const TriRec tr = tris[t];
+    Box b;
+    tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
+    uint32_t mn[3], mx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float bmin = g.first[k] - g.size[k] * 0.5f;  // grid.rs:264-265
+        mn[k] = snap_axis((b.lo[k] - bmin) / g.size[k], g.n[k]);
+        mx[k] = snap_axis((b.hi[k] - bmin) / g.size[k], g.n[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (mn[k] > 0 && cell_coord(g, k, mn[k]) > b.lo[k]) mn[k] -= 1;
+        if (mx[k] < g.n[k] - 1 && cell_coord(g, k, mx[k]) < b.hi[k]) mx[k] += 1;
+    }
+    SeedBox sb;
+    uint64_t cnt = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        sb.mn[k] = mn[k];
+        sb.ext[k] = mx[k] >= mn[k] ? mx[k] - mn[k] + 1 : 0;
+        cnt *= sb.ext[k];
+    }
+    boxes[t] = sb;
+    counts[t] = cnt;
+}
+
+template <int SIGN>
+__global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ tris, uint64_t nt, GridDev g,
+                                                    const SeedBox* __restrict__ boxes,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint64_t* __restrict__ total_ptr, uint64_t* keys,
+                                                    DevFlags* flags) {
+    const uint64_t total = *total_ptr;
+    bool nan_seen = false;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        // last triangle whose offset is <= p
+        uint64_t lo = 0, hi = nt - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (offsets[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint64_t t = lo;
+        const SeedBox sb = boxes[t];
+        uint64_t local = p - offsets[t];
+        uint64_t yz = (uint64_t)sb.ext[1] * sb.ext[2];
+        if (yz == 0 || local >= yz * sb.ext[0]) {  // cannot happen with a consistent prefix sum
+            atomicOr(&flags->internal, 1u);
+            continue;
+        }
+        uint32_t x = sb.mn[0] + (uint32_t)(local / yz);
+        uint64_t r = local % yz;
+        uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
+        uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
+        const TriRec tr = tris[t];
+        float d = cell_dist<SIGN>(g, x, y, z, tr);
+        nan_seen |= d != d;
+        if (key_valid(d)) {
+            uint64_t key = make_key<SIGN>(d, (uint32_t)t);
+            uint64_t* slot = &keys[cell_index(g, x, y, z)];
+            if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+        }
+    }
+    if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+}
+
+// ---------------------------------------------------------------- propagation
+
+__global__ void __launch_bounds__(256) k_frontier_init(const uint64_t* __restrict__ keys, uint64_t ncells,
+                                                       uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        bool seeded = keys[i] != KEY_EMPTY;
+        uint64_t m = __ballot(seeded);
+        if (!m) continue;
+        uint32_t lane = threadIdx.x & 63u;
+        uint32_t base = 0;
+        int leader = __ffsll((unsigned long long)m) - 1;
+        if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (seeded) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
+    }
+}
+
+struct FrontierItem {
+    uint32_t cell;
+    uint32_t tri;
+};
+
+__global__ void __launch_bounds__(256) k_frontier_snapshot(const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ count,
+                                                           uint32_t* __restrict__ next_count,
+                                                           const uint64_t* __restrict__ keys, uint8_t* changed,
+                                                           FrontierItem* __restrict__ items, int sign) {
+    const uint32_t n = *count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t c = list[i];
+        changed[c] = 0;
+        items[i] = FrontierItem{c, key_tri(keys[c], sign)};
+    }
+}
+
+template <int SIGN>
+__global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items,
+                                                            const uint32_t* __restrict__ count,
+                                                            const TriRec* __restrict__ tris, GridDev g,
+                                                            uint64_t* keys, uint8_t* changed,
+                                                            uint32_t* __restrict__ next_list, uint32_t* next_count,
+                                                            DevFlags* flags) {
+    const uint64_t work = (uint64_t)(*count) * 26u;
+    bool nan_seen = false;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < work;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t f = (uint32_t)(j / 26u);
+        uint32_t k = (uint32_t)(j % 26u);
+        k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
+        const int dx = (int)(k / 9u) - 1, dy = (int)((k / 3u) % 3u) - 1, dz = (int)(k % 3u) - 1;
+        const FrontierItem it = items[f];
+        const uint32_t cz = it.cell % g.n[2];
+        const uint32_t cy = (it.cell / g.n[2]) % g.n[1];
+        const uint32_t cx = it.cell / (g.n[1] * g.n[2]);
+        const int64_t x = (int64_t)cx + dx, y = (int64_t)cy + dy, z = (int64_t)cz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
+        const TriRec tr = tris[it.tri];
+        float d = cell_dist<SIGN>(g, (uint32_t)x, (uint32_t)y, (uint32_t)z, tr);
+        nan_seen |= d != d;
+        if (!key_valid(d)) continue;
+        const uint64_t key = make_key<SIGN>(d, it.tri);
+        const uint64_t ci = cell_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+        if (key >= __hip_atomic_load(&keys[ci], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
+        if (key < old) {
+            // first improver of this round appends the cell to the next frontier
+            uint32_t* word = (uint32_t*)((uintptr_t)&changed[ci] & ~(uintptr_t)3);
+            uint32_t shift = (uint32_t)(((uintptr_t)&changed[ci] & 3u) * 8u);
+            uint32_t prev = atomicOr(word, 1u << shift);
+            if (!((prev >> shift) & 1u)) next_list[atomicAdd(next_count, 1u)] = (uint32_t)ci;
+        }
+    }
+    if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+}
+
+// ---------------------------------------------------------------- raycast sign
+
+// Rows of axis a whose origin coordinate along perpendicular axis k lies in
+// [lo, hi]: a conservative index range, checked exactly per row afterwards.
+__device__ __forceinline__ void row_range(const GridDev& g, int k, float lo, float hi, uint32_t& i0, uint32_t& i1) {
+    float a = (lo - g.first[k]) / g.size[k];
+    float b = (hi - g.first[k]) / g.size[k];
+    float fmn = fminf(a, b), fmx = fmaxf(a, b);
+    float n1 = (float)(g.n[k] - 1);
+    // one extra row on each side absorbs the rounding of the division
+    float s = floorf(fmn) - 1.0f, e = ceilf(fmx) + 1.0f;
+    if (!(s >= 0.0f)) s = 0.0f;  // also NaN
+    if (!(e <= n1)) e = n1;
+    if (s > n1) s = n1;
+    if (e < 0.0f) e = 0.0f;
+    i0 = (uint32_t)s;
+    i1 = (uint32_t)e;
+}
+
+template <int AXIS>
+__device__ void scatter_axis(const TriRec& tr, const Box& b, const GridDev& g, uint32_t* __restrict__ bits) {
+    constexpr int K1 = AXIS == 0 ? 1 : 0;  // the row index is (K1, K2) with K2 fastest
+    constexpr int K2 = AXIS == 2 ? 1 : 2;
+    uint32_t a0, a1, c0, c1;
+    row_range(g, K1, b.lo[K1], b.hi[K1], a0, a1);
+    row_range(g, K2, b.lo[K2], b.hi[K2], c0, c1);
+    const float ox_along = cell_coord(g, AXIS, 0u);
+    for (uint32_t i = a0; i <= a1 && a0 <= a1; ++i) {
+        float u = cell_coord(g, K1, i);
+        if (!(b.lo[K1] <= u && u <= b.hi[K1])) continue;
+        for (uint32_t j = c0; j <= c1 && c0 <= c1; ++j) {
+            float v = cell_coord(g, K2, j);
+            if (!(b.lo[K2] <= v && v <= b.hi[K2])) continue;
+            float o[3];
+            o[AXIS] = ox_along;
+            o[K1] = u;
+            o[K2] = v;
+            if (!(b.hi[AXIS] >= o[AXIS])) continue;  // box entirely behind the origin
+            float t;
+            if (!ray_hit<AXIS>(o[0], o[1], o[2], tr, &t)) continue;
+            // generate/grid.rs:605-607: (t / size).floor() as usize, capped at n - 1
+            float q = floorf(t / g.size[AXIS]);
+            uint32_t kcell;
+            if (!(q > 0.0f)) kcell = 0;  // NaN and negatives saturate to 0
+            else if (q >= (float)(g.n[AXIS] - 1)) kcell = g.n[AXIS] - 1;
+            else kcell = (uint32_t)q;
+            uint64_t row = (uint64_t)i * g.n[K2] + j;
+            atomicXor(&bits[row * g.words[AXIS] + (kcell >> 5)], 1u << (kcell & 31u));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ray_scatter(const TriRec* __restrict__ tris, uint64_t nt, GridDev g,
+                                                     uint32_t* __restrict__ bx, uint32_t* __restrict__ by,
+                                                     uint32_t* __restrict__ bz) {
+    uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= 3 * nt) return;
+    uint64_t t = id / 3;
+    int axis = (int)(id % 3);
+    const TriRec tr = tris[t];
+    Box b;
+    tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
+    if (axis == 0) scatter_axis<0>(tr, b, g, bx);
+    else if (axis == 1) scatter_axis<1>(tr, b, g, by);
+    else scatter_axis<2>(tr, b, g, bz);
+}
+
+// Per row: bit i becomes the XOR of flip bits i..n-1 (the parity of hits whose
+// last counted cell is at or beyond i).
+__global__ void __launch_bounds__(256) k_ray_suffix(uint32_t* __restrict__ bits, uint64_t rows, uint32_t words) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    uint32_t* row = bits + r * words;
+    uint32_t carry = 0;
+    for (int w = (int)words - 1; w >= 0; --w) {
+        uint32_t x = row[w];
+        x ^= x >> 1;
+        x ^= x >> 2;
+        x ^= x >> 4;
+        x ^= x >> 8;
+        x ^= x >> 16;
+        if (carry) x = ~x;
+        row[w] = x;
+        carry = x & 1u;
+    }
+}
+
+template <int SIGN>
+__global__ void __launch_bounds__(256) k_grid_finish(const uint64_t* __restrict__ keys, GridDev g,
+                                                     const uint32_t* __restrict__ px,
+                                                     const uint32_t* __restrict__ py,
+                                                     const uint32_t* __restrict__ pz, float* __restrict__ out) {
+    const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float v = key_value(keys[i], SIGN);
+        if (SIGN == SDFX_SIGN_RAYCAST) {
+            uint32_t z = (uint32_t)(i % g.n[2]);
+            uint32_t y = (uint32_t)((i / g.n[2]) % g.n[1]);
+            uint32_t x = (uint32_t)(i / ((uint64_t)g.n[1] * g.n[2]));
+            uint32_t ox = (px[((uint64_t)y * g.n[2] + z) * g.words[0] + (x >> 5)] >> (x & 31u)) & 1u;
+            uint32_t oy = (py[((uint64_t)x * g.n[2] + z) * g.words[1] + (y >> 5)] >> (y & 31u)) & 1u;
+            uint32_t oz = (pz[((uint64_t)x * g.n[1] + y) * g.words[2] + (z >> 5)] >> (z & 31u)) & 1u;
+            if (ox + oy + oz >= 2u) v = -v;  // generate/grid.rs:633-638
+        }
+        out[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------- driver
+
+// One pinned word per thread for reading the frontier size back.
+static uint32_t* pinned_u32() {
+    struct Holder {
+        uint32_t* p = nullptr;
+        ~Holder() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    static thread_local Holder h;
+    if (!h.p) SDFX_HIP(hipHostMalloc((void**)&h.p, sizeof(uint32_t), hipHostMallocDefault));
+    return h.p;
+}
+
+static unsigned stride_blocks(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return (unsigned)(b ? b : 1);
+}
+
+template <int SIGN>
+static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float* d_out, DevFlags* d_flags,
+                     Arena& A) {
+    hipStream_t s = A.stream;
+    const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
+    uint64_t* keys = A.alloc<uint64_t>(ncells);
+    hipLaunchKernelGGL(k_fill_u64, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, KEY_EMPTY);
+
+    // 1. seeding
+    SeedBox* boxes = A.alloc<SeedBox>(nt);
+    uint64_t* counts = A.alloc<uint64_t>(nt);
+    uint64_t* total = A.alloc<uint64_t>(1);
+    hipLaunchKernelGGL(k_seed_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, g, boxes, counts);
+    exclusive_scan_u64(counts, counts, nt, total, A);
+    hipLaunchKernelGGL(k_seed_pairs<SIGN>, dim3(8192), dim3(256), 0, s, d_tris, nt, g, boxes, counts, total, keys,
+                       d_flags);
+    SDFX_HIP(hipGetLastError());
+
+    // 2. frontier rounds
+    uint32_t* lists[2] = {A.alloc<uint32_t>(ncells), A.alloc<uint32_t>(ncells)};
+    uint32_t* cnt = A.alloc<uint32_t>(2);
+    uint8_t* changed = A.alloc<uint8_t>((ncells + 3) & ~3ull);
+    FrontierItem* items = A.alloc<FrontierItem>(ncells);
+    hipLaunchKernelGGL(k_fill_u8, dim3(stride_blocks(ncells)), dim3(256), 0, s, changed, (ncells + 3) & ~3ull,
+                       (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u32g, dim3(1), dim3(64), 0, s, cnt, 2ull, 0u);
+    hipLaunchKernelGGL(k_frontier_init, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, lists[0], cnt);
+    SDFX_HIP(hipGetLastError());
+    uint32_t* h_cnt = pinned_u32();
+    const unsigned prop_blocks = 4096;
+    int cur = 0;
+    const int batch = 8;
+    // Every round strictly lowers some key, so the loop ends; the cap only turns
+    // an inconsistency into an error instead of a hang.
+    const uint64_t max_rounds = 64ull * (g.n[0] + g.n[1] + g.n[2]) + 1024;
+    {
+        for (uint64_t round = 0;;) {
+            if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
+            for (int b = 0; b < batch; ++b, ++round) {
+                int nxt = cur ^ 1;
+                hipLaunchKernelGGL(k_frontier_snapshot, dim3(2048), dim3(256), 0, s, lists[cur], cnt + cur,
+                                   cnt + nxt, keys, changed, items, (int)SIGN);
+                hipLaunchKernelGGL(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items,
+                                   cnt + cur, d_tris, g, keys, changed, lists[nxt], cnt + nxt, d_flags);
+                cur = nxt;
+            }
+            SDFX_HIP(hipGetLastError());
+            SDFX_HIP(hipMemcpyAsync(h_cnt, cnt + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            SDFX_HIP(hipStreamSynchronize(s));
+            if (*h_cnt == 0) break;
+        }
+    }
+
+    // 3. sign
+    uint32_t *bx = nullptr, *by = nullptr, *bz = nullptr;
+    if (SIGN == SDFX_SIGN_RAYCAST) {
+        uint64_t rows[3] = {(uint64_t)g.n[1] * g.n[2], (uint64_t)g.n[0] * g.n[2], (uint64_t)g.n[0] * g.n[1]};
+        uint32_t* bufs[3];
+        for (int a = 0; a < 3; ++a) {
+            uint64_t nw = rows[a] * g.words[a];
+            bufs[a] = A.alloc<uint32_t>(nw);
+            hipLaunchKernelGGL(k_fill_u32g, dim3(stride_blocks(nw)), dim3(256), 0, s, bufs[a], nw, 0u);
+        }
+        bx = bufs[0];
+        by = bufs[1];
+        bz = bufs[2];
+        hipLaunchKernelGGL(k_ray_scatter, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, d_tris, nt, g, bx, by, bz);
+        for (int a = 0; a < 3; ++a)
+            hipLaunchKernelGGL(k_ray_suffix, dim3(blocks_for(rows[a], 256)), dim3(256), 0, s, bufs[a], rows[a],
+                               g.words[a]);
+        SDFX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_grid_finish<SIGN>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, bx, by, bz,
+                       d_out);
+    SDFX_HIP(hipGetLastError());
+}
+
+void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
+              DevFlags* d_flags, Arena& A) {
+    const uint64_t ncells = gp.count[0] * gp.count[1] * gp.count[2];
+    if (!ncells) return;
+    GridDev g;
+    for (int k = 0; k < 3; ++k) {
+        g.first[k] = gp.first[k];
+        g.size[k] = gp.size[k];
+        g.n[k] = (uint32_t)gp.count[k];
+        g.words[k] = (uint32_t)((gp.count[k] + 31) / 32);
+    }
+    if (nt == 0) {
+        // no triangle: every cell keeps f32::MAX (grid.rs:137-143), no ray hits
+        hipLaunchKernelGGL(k_fill_u32g, dim3(stride_blocks(ncells)), dim3(256), 0, A.stream, (uint32_t*)d_out, ncells,
+                           fbits(FLT_MAX));
+        SDFX_HIP(hipGetLastError());
+        return;
+    }
+    if (sign == SDFX_SIGN_NORMAL) grid_run<SDFX_SIGN_NORMAL>(d_tris, nt, g, d_out, d_flags, A);
+    else grid_run<SDFX_SIGN_RAYCAST>(d_tris, nt, g, d_out, d_flags, A);
+}
+
+}  // namespace sdfx

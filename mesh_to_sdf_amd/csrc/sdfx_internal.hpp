@@ -1,0 +1,100 @@
+// sdfx_internal.hpp — interfaces between libsdfx's translation units.
+#pragma once
+
+#include "sdfx_geo.hpp"
+#include "sdfx_runtime.hpp"
+
+namespace sdfx {
+
+// ---------------------------------------------------------------- mesh prep
+
+// Triangle count of a topology (lib.rs:193-215), or the count error the
+// reference's asserts raise.
+sdfx_status topology_triangle_count(uint64_t n_vertices, bool has_indices, uint64_t n_indices,
+                                    sdfx_topology topology, uint64_t* n_tris);
+
+// Decodes the topology on the device and writes one TriRec per triangle in
+// topology order. Indices >= n_vertices raise flags->index_out_of_range (the
+// triangle then reads a clamped vertex so no access leaves the buffer).
+void gather_triangles(const float* d_vertices, uint64_t n_vertices, const uint32_t* d_indices,
+                      sdfx_topology topology, uint64_t n_tris, TriRec* d_tris, DevFlags* d_flags,
+                      hipStream_t stream);
+
+// Raises flags->non_finite_query if any coordinate is NaN or infinite (lib.rs:374-376).
+void check_queries_finite(const float* d_queries, uint64_t n_queries, DevFlags* d_flags, hipStream_t stream);
+
+// ---------------------------------------------------------------- None path
+
+// generate/generic/default.rs:22-85: every query against every triangle, in
+// triangle order. d_out receives the signed distances.
+void none_query(const TriRec* d_tris, uint64_t n_tris, const float* d_queries, uint64_t n_queries,
+                sdfx_sign_method sign, float* d_out, DevFlags* d_flags, Arena& arena);
+
+// ---------------------------------------------------------------- LBVH
+
+// Internal node: both children's boxes plus child links. A link >= 0 is an
+// internal node, a link < 0 is the leaf ~link (position in Morton order).
+struct __attribute__((aligned(16))) BvhNode {
+    float lmin[3];
+    int32_t left;
+    float lmax[3];
+    int32_t right;
+    float rmin[3];
+    uint32_t pad0;
+    float rmax[3];
+    uint32_t pad1;
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+
+struct Lbvh {
+    uint64_t n = 0;          // leaves (triangles)
+    int32_t root = 0;        // root link (leaf ~0 when n == 1)
+    DevBuf<BvhNode> nodes;   // n - 1 internal nodes
+    DevBuf<TriRec> leaves;   // triangles in Morton order (TriRec::index keeps topology order)
+};
+
+// Builds the LBVH: centroid bounds, 30-bit Morton codes, stable LDS radix
+// sort, Karras hierarchy, bottom-up refit of the padded triangle boxes.
+void lbvh_build(const TriRec* d_tris, uint64_t n_tris, Lbvh& bvh, Arena& arena);
+
+enum QueryMode : int {
+    QM_RAYCAST_VOTE = 0,     // min distance (clamped to f32::MAX) + 3-ray vote: Bvh(Raycast)
+    QM_NORMAL_FOLD = 1,      // compare_distances fold: Bvh(Normal)
+    QM_NEAREST_SIGNED = 2,   // nearest by d^2, signed distance to it: Rtree
+    QM_NEAREST_VOTE = 3,     // nearest by d^2, distance to it + 3-ray vote: RtreeBvh
+};
+
+void lbvh_query(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, QueryMode mode, float* d_out,
+                DevFlags* d_flags, hipStream_t stream);
+
+// generic/rtree_bvh.rs:217-276: closest - query and the 3-ray inside vote.
+void lbvh_closest_vectors(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, float* d_vectors,
+                          uint8_t* d_inside, DevFlags* d_flags, hipStream_t stream);
+
+// ---------------------------------------------------------------- grid
+
+struct GridParams {
+    float first[3];
+    float size[3];
+    uint64_t count[3];
+};
+
+// generate/grid.rs:265-378 on the device: narrow-band seeding, frontier
+// propagation to a fixpoint, grid-aligned raycast sign (Raycast only).
+void grid_sdf(const TriRec* d_tris, uint64_t n_tris, const GridParams& g, sdfx_sign_method sign, float* d_out,
+              DevFlags* d_flags, Arena& arena);
+
+// ---------------------------------------------------------------- utilities
+
+// Fills n u32 words with v using a kernel (no memset node, so the call is safe
+// inside captured graphs).
+void fill_u32(uint32_t* d_ptr, uint64_t n, uint32_t v, hipStream_t stream);
+
+// ---------------------------------------------------------------- scans
+
+// Exclusive prefix sum of n values on the device (in place allowed); the total
+// goes to *d_total when it is not null. No host synchronisation.
+void exclusive_scan_u64(const uint64_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total, Arena& arena);
+void exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, uint32_t* d_total, Arena& arena);
+
+}  // namespace sdfx

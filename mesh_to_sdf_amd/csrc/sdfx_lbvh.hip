@@ -1,0 +1,685 @@
+// sdfx_lbvh.hip — device LBVH over the padded triangle boxes (geo.rs:4-22) and
+// the point-query kernels of the Bvh, Rtree and RtreeBvh paths.
+//
+// Build: centroid bounds -> 30-bit Morton codes -> stable LSD radix sort (8-bit
+// digits, LDS histograms, wave-ballot multisplit ranking) -> Karras 2012
+// hierarchy over (code, position) keys -> bottom-up refit with one atomic
+// visit counter per node. Each internal node stores both children's boxes, so
+// one 64-byte fetch decides both descents.
+//
+// Queries. The reference's R-tree (rstar 0.12) and BVH (bvh 0.10) searches
+// are both conservative, so their results are fixed by the triangles
+// themselves (generic/bvh.rs:131-176, generic/rtree.rs:147-154,
+// generic/rtree_bvh.rs:153-198):
+//  * nearest triangle = minimum FP32 d^2 (lowest topology index on exact ties);
+//  * each of the +X/+Y/+Z ray counts = number of triangles the aligned test
+//    hits, since a hit point lies inside the triangle's padded box.
+// Nodes are pruned only when their box distance is clearly above the best d^2
+// (2^-16 relative slack) and ray descent keeps every box the ray meets, so the
+// device results equal the exhaustive ones.
+#include "sdfx_internal.hpp"
+
+namespace sdfx {
+
+// ------------------------------------------------------------------ bounds
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = fbits(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return bitsf((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__device__ __forceinline__ void tri_centroid(const TriRec& t, float* c) {
+    Box b;
+    tri_bbox(t.ax, t.ay, t.az, t.bx, t.by, t.bz, t.cx, t.cy, t.cz, b);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = 0.5f * (b.lo[k] + b.hi[k]);
+}
+
+__global__ void k_init_bounds(uint32_t* ob) {
+    if (threadIdx.x < 3) ob[threadIdx.x] = 0xFFFFFFFFu;
+    else if (threadIdx.x < 6) ob[threadIdx.x] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_centroid_bounds(const TriRec* __restrict__ tris, uint64_t n,
+                                                         uint32_t* __restrict__ ob) {
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float c[3];
+        tri_centroid(tris[i], c);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uint32_t o = f2ord(c[k]);
+            mn[k] = min(mn[k], o);
+            mx[k] = max(mx[k], o);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off, 64));
+            mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&ob[k], mn[k]);
+            atomicMax(&ob[3 + k], mx[k]);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_morton(const TriRec* __restrict__ tris, uint64_t n,
+                                                const uint32_t* __restrict__ ob, uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float c[3];
+    tri_centroid(tris[i], c);
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lo = ord2f(ob[k]), hi = ord2f(ob[3 + k]);
+        float ext = hi - lo;
+        float u = ext > 0.0f ? (c[k] - lo) / ext : 0.0f;
+        float s = u * 1024.0f;
+        q[k] = s <= 0.0f ? 0u : (s >= 1023.0f ? 1023u : (uint32_t)s);  // NaN -> 0
+    }
+    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    vals[i] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------------ radix sort
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n, int shift,
+                                                        uint32_t* __restrict__ hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        uint64_t i = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items are ranked in position order. Within one round of 256
+// items, a lane's rank among equal digits of its wave comes from 8 ballots
+// (multisplit); earlier waves' counts come from LDS; earlier rounds' counts
+// from the running per-digit offsets.
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint32_t* __restrict__ keys_in,
+                                                           const uint32_t* __restrict__ vals_in, uint64_t n,
+                                                           int shift, const uint32_t* __restrict__ offsets,
+                                                           uint32_t nblocks, uint32_t* __restrict__ keys_out,
+                                                           uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wave_cnt[RS_WAVES][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    run[tid] = offsets[(uint64_t)tid * nblocks + blockIdx.x];
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) wave_cnt[w][tid] = 0;
+        __syncthreads();
+        uint64_t i = base + (uint64_t)r * RS_THREADS + tid;
+        bool valid = i < n;
+        uint32_t key = valid ? keys_in[i] : 0u;
+        uint32_t val = valid ? vals_in[i] : 0u;
+        uint32_t digit = (key >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (digit >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank_in_wave = (uint32_t)__popcll(peers & lt_mask);
+        // the highest lane of each peer group publishes the group's count
+        if (valid && (peers >> lane) == 1ull) wave_cnt[wave][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t before = 0;
+            for (int w = 0; w < wave; ++w) before += wave_cnt[w][digit];
+            uint32_t dst = run[digit] + before + rank_in_wave;
+            if (dst < n) {
+                keys_out[dst] = key;
+                vals_out[dst] = val;
+            }
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) tot += wave_cnt[w][tid];
+        run[tid] += tot;
+        __syncthreads();
+    }
+}
+
+// Sorts (keys, vals) by the low `bits` bits of keys; stable. Result ends in keys/vals.
+static void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits, Arena& A) {
+    if (n <= 1) return;
+    uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+    uint32_t* k2 = A.alloc<uint32_t>(n);
+    uint32_t* v2 = A.alloc<uint32_t>(n);
+    uint32_t* hist = A.alloc<uint32_t>((uint64_t)256 * nblocks);
+    uint32_t *ki = keys, *vi = vals, *ko = k2, *vo = v2;
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, A.stream, ki, n, shift, hist, nblocks);
+        exclusive_scan_u32(hist, hist, (uint64_t)256 * nblocks, nullptr, A);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_THREADS), 0, A.stream, ki, vi, n, shift, hist,
+                           nblocks, ko, vo);
+        SDFX_HIP(hipGetLastError());
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    if (ki != keys) {
+        SDFX_HIP(hipMemcpyAsync(keys, ki, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, A.stream));
+        SDFX_HIP(hipMemcpyAsync(vals, vi, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, A.stream));
+    }
+}
+
+// ------------------------------------------------------------------ Karras build
+
+// Common-prefix length of sorted keys i and j; equal codes fall back to the
+// positions so every key is distinct (Karras 2012, section 4).
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = keys[i], b = keys[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((uint32_t)(i ^ j));
+}
+
+__global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ keys, int64_t n,
+                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
+                                                int32_t* __restrict__ parent_int, int32_t* __restrict__ parent_leaf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int64_t j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int64_t s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    int32_t lc = lo == gamma ? ~(int32_t)gamma : (int32_t)gamma;
+    int32_t rc = hi == gamma + 1 ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    if (lc < 0) parent_leaf[~lc] = (int32_t)i;
+    else parent_int[lc] = (int32_t)i;
+    if (rc < 0) parent_leaf[~rc] = (int32_t)i;
+    else parent_int[rc] = (int32_t)i;
+    if (i == 0) parent_int[0] = -1;
+}
+
+__global__ void __launch_bounds__(256) k_leaves(const TriRec* __restrict__ tris, const uint32_t* __restrict__ order,
+                                                uint64_t n, TriRec* __restrict__ leaves, Box* __restrict__ leaf_box) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    TriRec t = tris[order[i]];
+    leaves[i] = t;
+    Box b;
+    tri_bbox(t.ax, t.ay, t.az, t.bx, t.by, t.bz, t.cx, t.cy, t.cz, b);
+    leaf_box[i] = b;
+}
+
+// A box another lane wrote before its release fence: read around the L1.
+__device__ __forceinline__ Box load_box_coherent(const Box* b) {
+    const volatile float* f = (const volatile float*)b;
+    Box r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = f[k];
+        r.hi[k] = f[3 + k];
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_refit(uint64_t n, const int32_t* __restrict__ parent_leaf,
+                                               const int32_t* __restrict__ parent_int,
+                                               const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                               const Box* __restrict__ leaf_box, Box* node_box, uint32_t* visit,
+                                               BvhNode* __restrict__ nodes) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t p = parent_leaf[i];
+    for (int level = 0; p >= 0 && level < 128; ++level) {  // depth <= 64 in a Karras tree
+        __threadfence();
+        if (atomicAdd(&visit[p], 1u) == 0u) return;  // the sibling subtree is not finished yet
+        __threadfence();
+        int32_t l = left[p], r = right[p];
+        Box lb = l < 0 ? leaf_box[~l] : load_box_coherent(&node_box[l]);
+        Box rb = r < 0 ? leaf_box[~r] : load_box_coherent(&node_box[r]);
+        Box u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u.lo[k] = fminf(lb.lo[k], rb.lo[k]);
+            u.hi[k] = fmaxf(lb.hi[k], rb.hi[k]);
+        }
+        node_box[p] = u;
+        BvhNode nd;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            nd.lmin[k] = lb.lo[k];
+            nd.lmax[k] = lb.hi[k];
+            nd.rmin[k] = rb.lo[k];
+            nd.rmax[k] = rb.hi[k];
+        }
+        nd.left = l;
+        nd.right = r;
+        nd.pad0 = 0;
+        nd.pad1 = 0;
+        nodes[p] = nd;
+        p = parent_int[p];
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
+    hipStream_t s = A.stream;
+    bvh.n = n;
+    if (n == 0) return;
+    // ray-traversal stack entries pack a node index with 3 axis bits
+    if (n >= (1ull << 29)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "too many triangles for one LBVH");
+    bvh.leaves.reset(n);
+    bvh.nodes.reset(n > 1 ? n - 1 : 1);
+    uint32_t* bounds = A.alloc<uint32_t>(6);
+    uint32_t* keys = A.alloc<uint32_t>(n);
+    uint32_t* vals = A.alloc<uint32_t>(n);
+    Box* leaf_box = A.alloc<Box>(n);
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, bounds);
+    unsigned bb = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(bb > 1024 ? 1024 : bb), dim3(256), 0, s, d_tris, n, bounds);
+    hipLaunchKernelGGL(k_morton, dim3(bb), dim3(256), 0, s, d_tris, n, bounds, keys, vals);
+    SDFX_HIP(hipGetLastError());
+    radix_sort_pairs(keys, vals, n, 30, A);
+    hipLaunchKernelGGL(k_leaves, dim3(bb), dim3(256), 0, s, d_tris, vals, n, bvh.leaves.get(), leaf_box);
+    SDFX_HIP(hipGetLastError());
+    if (n == 1) {
+        bvh.root = ~0;
+        return;
+    }
+    bvh.root = 0;
+    uint64_t ni = n - 1;
+    int32_t* left = A.alloc<int32_t>(ni);
+    int32_t* right = A.alloc<int32_t>(ni);
+    int32_t* parent_int = A.alloc<int32_t>(ni);
+    int32_t* parent_leaf = A.alloc<int32_t>(n);
+    Box* node_box = A.alloc<Box>(ni);
+    uint32_t* visit = A.alloc<uint32_t>(ni);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(ni, 256)), dim3(256), 0, s, visit, ni);
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(ni, 256)), dim3(256), 0, s, keys, (int64_t)n, left, right,
+                       parent_int, parent_leaf);
+    hipLaunchKernelGGL(k_refit, dim3(bb), dim3(256), 0, s, n, parent_leaf, parent_int, left, right, leaf_box,
+                       node_box, visit, bvh.nodes.get());
+    SDFX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ traversal
+
+constexpr int STACK_DEPTH = 64;  // Karras depth <= 64 (32 code bits + 32 position bits)
+constexpr float PRUNE_SLACK = 1.0f - 1.0f / 65536.0f;
+
+struct Nearest {
+    float d2;
+    uint32_t index;  // topology index (tie-break)
+    int32_t leaf;    // Morton position, -1 if none
+};
+
+__device__ __forceinline__ void visit_leaf_nearest(const TriRec* __restrict__ leaves, int32_t leaf, float px,
+                                                   float py, float pz, Nearest& best) {
+    const TriRec t = leaves[leaf];
+    float d2 = tri_dist2(px, py, pz, t);
+    if (d2 < best.d2 || (d2 == best.d2 && t.index < best.index)) {
+        best.d2 = d2;
+        best.index = t.index;
+        best.leaf = leaf;
+    }
+}
+
+// Minimum d^2 over all triangles, closest child first.
+__device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
+                               float px, float py, float pz, uint32_t max_visits, bool& overflow) {
+    Nearest best{INFINITY, 0xFFFFFFFFu, -1};
+    if (root < 0) {
+        visit_leaf_nearest(leaves, ~root, px, py, pz, best);
+        return best;
+    }
+    int32_t stack_node[STACK_DEPTH];
+    float stack_d[STACK_DEPTH];
+    int sp = 0;
+    int32_t node = root;
+    for (uint32_t visits = 0;; ++visits) {
+        if (visits > max_visits) {  // a tree visits each node once; more means a malformed tree
+            overflow = true;
+            break;
+        }
+        const BvhNode nd = nodes[node];
+        float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
+        float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
+        bool gl = dl * PRUNE_SLACK <= best.d2;
+        bool gr = dr * PRUNE_SLACK <= best.d2;
+        if (gl && nd.left < 0) {
+            visit_leaf_nearest(leaves, ~nd.left, px, py, pz, best);
+            gl = false;
+            gr = dr * PRUNE_SLACK <= best.d2;
+        }
+        if (gr && nd.right < 0) {
+            visit_leaf_nearest(leaves, ~nd.right, px, py, pz, best);
+            gr = false;
+        }
+        if (gl && gr) {
+            bool left_first = dl <= dr;
+            if (sp < STACK_DEPTH) {
+                stack_node[sp] = left_first ? nd.right : nd.left;
+                stack_d[sp] = left_first ? dr : dl;
+                sp++;
+            } else {
+                overflow = true;
+            }
+            node = left_first ? nd.left : nd.right;
+            continue;
+        }
+        if (gl) {
+            node = nd.left;
+            continue;
+        }
+        if (gr) {
+            node = nd.right;
+            continue;
+        }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if (stack_d[sp] * PRUNE_SLACK <= best.d2) {
+                node = stack_node[sp];
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+// Bvh(Normal) (generic/bvh.rs:117-129): min_dist starts at f32::MAX and takes
+// each candidate's signed distance when compare_distances(min_dist, d) is
+// Greater. Nodes are skipped only when their box is farther than any value
+// that could tie with (ulps 2 / eps 1e-6) or beat |min_dist|.
+__device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves,
+                                 int32_t root, float px, float py, float pz, uint32_t max_visits, bool& nan_seen,
+                                 bool& overflow) {
+    float acc = FLT_MAX;
+    auto bound2 = [&](void) {
+        float b = fabsf(acc) * (1.0f + 1.0f / 1048576.0f) + 4e-6f;
+        return b * b * (1.0f + 1.0f / 1048576.0f);
+    };
+    auto visit = [&](int32_t leaf) {
+        float d = tri_signed_dist(px, py, pz, leaves[leaf]);
+        int c = compare_distances(acc, d);
+        nan_seen |= c == 2;
+        if (c == 1) acc = d;
+    };
+    if (root < 0) {
+        visit(~root);
+        return acc;
+    }
+    int32_t stack_node[STACK_DEPTH];
+    float stack_d[STACK_DEPTH];
+    int sp = 0;
+    int32_t node = root;
+    for (uint32_t visits = 0;; ++visits) {
+        if (visits > max_visits) {  // a tree visits each node once; more means a malformed tree
+            overflow = true;
+            break;
+        }
+        const BvhNode nd = nodes[node];
+        float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
+        float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
+        float b2 = bound2();
+        bool gl = dl <= b2, gr = dr <= b2;
+        if (gl && nd.left < 0) {
+            visit(~nd.left);
+            gl = false;
+            gr = dr <= bound2();
+        }
+        if (gr && nd.right < 0) {
+            visit(~nd.right);
+            gr = false;
+        }
+        if (gl && gr) {
+            bool left_first = dl <= dr;
+            if (sp < STACK_DEPTH) {
+                stack_node[sp] = left_first ? nd.right : nd.left;
+                stack_d[sp] = left_first ? dr : dl;
+                sp++;
+            } else {
+                overflow = true;
+            }
+            node = left_first ? nd.left : nd.right;
+            continue;
+        }
+        if (gl) {
+            node = nd.left;
+            continue;
+        }
+        if (gr) {
+            node = nd.right;
+            continue;
+        }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if (stack_d[sp] <= bound2()) {
+                node = stack_node[sp];
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return acc;
+}
+
+// Axes (bit k = +k ray) whose ray from p can meet the box: the two
+// perpendicular coordinates inside, the box not entirely behind p.
+__device__ __forceinline__ uint32_t ray_axes(const float* lo, const float* hi, float px, float py, float pz) {
+    bool ix = lo[0] <= px && px <= hi[0];
+    bool iy = lo[1] <= py && py <= hi[1];
+    bool iz = lo[2] <= pz && pz <= hi[2];
+    uint32_t m = 0;
+    if (iy && iz && hi[0] >= px) m |= 1u;
+    if (iz && ix && hi[1] >= py) m |= 2u;
+    if (ix && iy && hi[2] >= pz) m |= 4u;
+    return m;
+}
+
+__device__ __forceinline__ void ray_leaf(const TriRec* __restrict__ leaves, int32_t leaf, uint32_t axes, float px,
+                                         float py, float pz, uint32_t* cnt) {
+    const TriRec t = leaves[leaf];
+    if ((axes & 1u) && ray_hit<0>(px, py, pz, t, nullptr)) cnt[0]++;
+    if ((axes & 2u) && ray_hit<1>(px, py, pz, t, nullptr)) cnt[1]++;
+    if ((axes & 4u) && ray_hit<2>(px, py, pz, t, nullptr)) cnt[2]++;
+}
+
+// The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198: the
+// three rays share one traversal; each stack entry carries its live axes.
+__device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
+                                float px, float py, float pz, uint32_t max_visits, bool& overflow) {
+    uint32_t cnt[3] = {0u, 0u, 0u};
+    if (root < 0) {
+        ray_leaf(leaves, ~root, 7u, px, py, pz, cnt);
+    } else {
+        uint32_t stack[STACK_DEPTH];
+        int sp = 0;
+        int32_t node = root;
+        uint32_t axes = 7u;
+        for (uint32_t visits = 0;; ++visits) {
+            if (visits > max_visits) {
+                overflow = true;
+                break;
+            }
+            const BvhNode nd = nodes[node];
+            uint32_t al = axes & ray_axes(nd.lmin, nd.lmax, px, py, pz);
+            uint32_t ar = axes & ray_axes(nd.rmin, nd.rmax, px, py, pz);
+            if (al && nd.left < 0) {
+                ray_leaf(leaves, ~nd.left, al, px, py, pz, cnt);
+                al = 0;
+            }
+            if (ar && nd.right < 0) {
+                ray_leaf(leaves, ~nd.right, ar, px, py, pz, cnt);
+                ar = 0;
+            }
+            if (al && ar) {
+                if (sp < STACK_DEPTH) stack[sp++] = ((uint32_t)nd.right << 3) | ar;
+                else overflow = true;
+                node = nd.left;
+                axes = al;
+                continue;
+            }
+            if (al) {
+                node = nd.left;
+                axes = al;
+                continue;
+            }
+            if (ar) {
+                node = nd.right;
+                axes = ar;
+                continue;
+            }
+            if (sp == 0) break;
+            uint32_t e = stack[--sp];
+            node = (int32_t)(e >> 3);
+            axes = e & 7u;
+        }
+    }
+    int insides = (int)(cnt[0] & 1u) + (int)(cnt[1] & 1u) + (int)(cnt[2] & 1u);
+    return insides > 1;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ nodes,
+                                                    const TriRec* __restrict__ leaves, int32_t root,
+                                                    uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
+                                                    float* __restrict__ out, DevFlags* flags) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+    bool overflow = false;
+    if (MODE == QM_NORMAL_FOLD) {
+        bool nan_seen = false;
+        float acc = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
+        if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+        if (overflow) atomicOr(&flags->internal, 1u);
+        out[i] = acc;
+        return;
+    }
+    Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow);
+    if (MODE == QM_NEAREST_SIGNED) {
+        out[i] = nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
+    } else {
+        float d = sqrtf(nb.d2);
+        if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
+        out[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow) ? -d : d;
+    }
+    if (overflow) atomicOr(&flags->internal, 1u);
+}
+
+static uint32_t visit_cap(const Lbvh& bvh) { return (uint32_t)(2 * bvh.n + 64); }
+
+void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, float* d_out, DevFlags* d_flags,
+                hipStream_t s) {
+    if (!nq || !bvh.n) return;
+    unsigned blocks = blocks_for(nq, 256);
+    switch (mode) {
+        case QM_RAYCAST_VOTE:
+            hipLaunchKernelGGL(k_lbvh_query<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+            break;
+        case QM_NORMAL_FOLD:
+            hipLaunchKernelGGL(k_lbvh_query<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+            break;
+        case QM_NEAREST_SIGNED:
+            hipLaunchKernelGGL(k_lbvh_query<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+            break;
+        case QM_NEAREST_VOTE:
+            hipLaunchKernelGGL(k_lbvh_query<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+            break;
+    }
+    SDFX_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restrict__ nodes,
+                                                         const TriRec* __restrict__ leaves, int32_t root,
+                                                         uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
+                                                         float* __restrict__ vec, uint8_t* __restrict__ inside,
+                                                         DevFlags* flags) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+    bool overflow = false;
+    Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow);
+    float vx = NAN, vy = NAN, vz = NAN;
+    if (nb.leaf >= 0) {
+        float dx, dy, dz, cx, cy, cz;
+        closest_diff(px, py, pz, leaves[nb.leaf], dx, dy, dz, cx, cy, cz);
+        vx = cx - px;  // closest_point.sub(point)
+        vy = cy - py;
+        vz = cz - pz;
+    }
+    vec[3 * i] = vx;
+    vec[3 * i + 1] = vy;
+    vec[3 * i + 2] = vz;
+    inside[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow) ? 1 : 0;
+    if (overflow) atomicOr(&flags->internal, 1u);
+}
+
+void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float* d_vec, uint8_t* d_inside,
+                          DevFlags* d_flags, hipStream_t s) {
+    if (!nq || !bvh.n) return;
+    hipLaunchKernelGGL(k_closest_vectors, dim3(blocks_for(nq, 256)), dim3(256), 0, s, bvh.nodes.get(),
+                       bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_vec, d_inside, d_flags);
+    SDFX_HIP(hipGetLastError());
+}
+
+}  // namespace sdfx

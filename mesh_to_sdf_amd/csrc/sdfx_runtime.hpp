@@ -1,0 +1,120 @@
+// sdfx_runtime.hpp — host-side runtime shared by the libsdfx translation units:
+// status/error plumbing, per-thread streams and a caching device allocator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sdfx.h"
+
+namespace sdfx {
+
+// Thrown inside the library, converted to a status at the C boundary.
+struct Error : std::runtime_error {
+    sdfx_status status;
+    Error(sdfx_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+void set_last_error(const std::string& msg);
+void clear_last_error();
+const std::string& last_error();
+
+#define SDFX_HIP(expr)                                                                                 \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::sdfx::Error(e_ == hipErrorOutOfMemory ? SDFX_ERR_OUT_OF_MEMORY : SDFX_ERR_DEVICE,  \
+                                std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+// Stream used by host-pointer entry points: one per calling thread, created
+// lazily with the default priority.
+hipStream_t thread_stream();
+
+// Caching device allocator. Blocks are returned to a per-device free list and
+// reused by later calls; memory is released only at process exit.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+// RAII owner of one device block.
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { reset(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), n(o.n) {
+        o.ptr = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            n = o.n;
+            o.ptr = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void reset(size_t count) {
+        release();
+        n = count;
+        ptr = count ? static_cast<T*>(dev_alloc(count * sizeof(T))) : nullptr;
+    }
+    void release() {
+        if (ptr) dev_free(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    T* get() const { return ptr; }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Temporary device memory of one call. Blocks go back to the pool only after
+// the stream is synchronised (in the destructor), so an exception thrown while
+// kernels are in flight cannot hand their memory to another call. Declare the
+// Arena after every DevBuf a call owns, so it is destroyed (and syncs) first.
+struct Arena {
+    hipStream_t stream;
+    std::vector<void*> blocks;
+    explicit Arena(hipStream_t s) : stream(s) {}
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    template <typename T>
+    T* alloc(size_t count) {
+        void* p = dev_alloc((count ? count : 1) * sizeof(T));
+        blocks.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~Arena() {
+        (void)hipStreamSynchronize(stream);
+        for (void* p : blocks) dev_free(p);
+    }
+};
+
+// Device-side error flags of one call, read back once at the end.
+struct DevFlags {
+    uint32_t non_finite_query;
+    uint32_t index_out_of_range;
+    uint32_t nan_distance;
+    uint32_t internal;  // a device-side capacity or consistency check failed
+};
+
+int current_device();
+
+// Launch helper: blocks for n items with `threads` per block.
+inline unsigned blocks_for(uint64_t n, unsigned threads) {
+    uint64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b ? b : 1);
+}
+
+}  // namespace sdfx

@@ -1,0 +1,129 @@
+/*
+ * sdf_oracle.h — CPU restatement of mesh_to_sdf's hot path (generate_sdf and
+ * generate_grid_sdf), written in plain C from the reference's Rust sources.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker for the HIP
+ * product path and the CPU baseline timed beside it. Only tests/, the smoke()
+ * entry of __graft_entry__.py and bench.py's cpu_baseline leg may load it.
+ * The product (libsdfx) never links, loads or falls back to it.
+ *
+ * Numerics: FP32 evaluated in the reference's operation order, compiled with
+ * -ffp-contract=off -fno-fast-math on x86-64 (SSE, no excess precision), so
+ * every expression rounds exactly as the Rust code does (Rust never contracts
+ * a*b+c into an FMA and its sqrt and / are IEEE correctly rounded).
+ *
+ * Status codes are numerically identical to include/sdfx.h's sdfx_status.
+ */
+#ifndef SDF_ORACLE_H
+#define SDF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_OK = 0,
+    ORC_EMPTY_RESULT = 1,
+    ORC_ERR_NON_FINITE_QUERY = -1,
+    ORC_ERR_INDEX_COUNT = -2,
+    ORC_ERR_VERTEX_COUNT = -3,
+    ORC_ERR_INDEX_OUT_OF_RANGE = -4,
+    ORC_ERR_EMPTY_MESH = -5,
+    ORC_ERR_NAN_DISTANCE = -6,
+    ORC_ERR_INVALID_ARGUMENT = -7,
+    ORC_ERR_INVALID_GRID = -10
+};
+
+enum { ORC_TOPO_LIST = 0, ORC_TOPO_STRIP = 1 };
+enum { ORC_SIGN_RAYCAST = 0, ORC_SIGN_NORMAL = 1 };
+enum { ORC_ACCEL_NONE = 0, ORC_ACCEL_BVH = 1, ORC_ACCEL_RTREE = 2, ORC_ACCEL_RTREE_BVH = 3 };
+/* grid propagation schedules */
+enum { ORC_GRID_SEQUENTIAL = 0, ORC_GRID_THREADED = 1, ORC_GRID_FRONTIER = 2 };
+
+/* ---- geometry (geo.rs) ---- */
+void orc_triangle_bounding_box(const float* a, const float* b, const float* c, float* lo, float* hi);
+void orc_closest_point_triangle(const float* p, const float* a, const float* b, const float* c, float* out);
+void orc_closest_point_segment(const float* p, const float* a, const float* b, float* out);
+float orc_point_triangle_distance(const float* p, const float* a, const float* b, const float* c);
+float orc_point_triangle_distance2(const float* p, const float* a, const float* b, const float* c);
+float orc_point_triangle_signed_distance(const float* p, const float* a, const float* b, const float* c);
+int orc_ray_triangle_intersection_aligned(const float* origin, const float* a, const float* b, const float* c,
+                                          int axis, float* t_out);
+int orc_ray_triangle_intersection_generic(const float* origin, const float* dir, const float* a, const float* b,
+                                          const float* c, float* t_out);
+float orc_baseline_point_triangle_distance(const float* p, const float* a, const float* b, const float* c);
+
+/* ---- comparisons (lib.rs, float-cmp 0.9.0) ---- */
+int orc_approx_eq_f32(float a, float b, int32_t ulps, float epsilon);
+/* -1 Less, 0 Equal, 1 Greater, 2 = the reference panics ("NaN distance") */
+int orc_compare_distances(float a, float b);
+
+/* ---- topology (lib.rs:193-215) ---- */
+int orc_decode_topology(uint64_t n_vertices, const uint32_t* indices, uint64_t n_indices, int topology,
+                        uint32_t* tris_out, uint64_t tris_capacity, uint64_t* n_tris_out);
+
+/* ---- grid arithmetic (grid.rs) ---- */
+void orc_grid_from_bounding_box(const float* bmin, const float* bmax, const uint64_t* count, float* first,
+                                float* size);
+void orc_grid_cell_center(const float* first, const float* size, const uint64_t* cell, float* out);
+/* returns 1 for SnapResult::Inside, 0 for Outside; -1 when the reference panics */
+int orc_grid_snap(const float* first, const float* size, const uint64_t* count, const float* p, uint64_t* cell);
+
+/* ---- entry points ---- */
+int orc_generate_sdf(const float* vertices, uint64_t n_vertices, const uint32_t* indices, uint64_t n_indices,
+                     int topology, const float* queries, uint64_t n_queries, int accel, int sign, float* out,
+                     int n_threads, uint64_t* out_len);
+
+typedef struct {
+    uint64_t seeded_cells;   /* cells holding a value after seeding */
+    uint64_t seed_updates;   /* successful preheap updates */
+    uint64_t pops;           /* heap pops during propagation */
+    uint64_t rays;           /* rays cast */
+    uint64_t ray_candidates; /* candidates returned by BVH traversal */
+    uint64_t rounds;         /* frontier schedule: propagation rounds */
+    uint64_t evals;          /* frontier schedule: neighbour evaluations */
+} orc_grid_stats;
+
+/* schedule ORC_GRID_SEQUENTIAL: preheap in triangle order, the sorted heap split
+ * round-robin into n_heaps heaps run one after another (one legal schedule of the
+ * reference's concurrent threads; n_heaps=1 is the single-thread reference).
+ * schedule ORC_GRID_THREADED: n_heaps OS threads run concurrently, as the
+ * reference does (result depends on timing).
+ * schedule ORC_GRID_FRONTIER: not one of the reference's schedules. The same
+ * relaxation rule (a cell's triangle is tried on its 27 neighbours, kept on
+ * strict improvement) run as rounds over the frontier of cells improved in the
+ * previous round, each cell holding the minimum (distance bits, triangle index)
+ * key. It is deterministic and is the schedule the device pipeline runs, so it
+ * pins the device kernels bit-exactly; its distance from the reference's own
+ * schedules is measured separately. */
+int orc_generate_grid_sdf(const float* vertices, uint64_t n_vertices, const uint32_t* indices, uint64_t n_indices,
+                          int topology, const float* first_cell, const float* cell_size,
+                          const uint64_t* cell_count, int sign, float* out, int n_heaps, int schedule,
+                          float* seed_out, orc_grid_stats* stats);
+
+/* Vectors from each query to its closest point on the nearest triangle, plus the
+ * 3-ray inside vote (generate/generic/rtree_bvh.rs:217-276). */
+int orc_query_vector_to_closest_point(const float* vertices, uint64_t n_vertices, const uint32_t* indices,
+                                      uint64_t n_indices, int topology, const float* queries, uint64_t n_queries,
+                                      float* vec_out, uint8_t* inside_out, int n_threads);
+
+/* ---- helpers for tests ---- */
+/* exact FP32 nearest triangle (lowest index on ties) by brute force */
+void orc_nearest_bruteforce(const float* vertices, const uint32_t* tris, uint64_t n_tris, const float* points,
+                            uint64_t n_points, float* d2_out, uint32_t* tri_out, int n_threads);
+/* FP64 brute-force unsigned distance (Ericson's closest point evaluated in double) */
+void orc_fp64_distance(const float* vertices, const uint32_t* tris, uint64_t n_tris, const float* points,
+                       uint64_t n_points, double* out, int n_threads);
+/* number of +axis ray crossings per point over all triangles (brute force) */
+void orc_ray_parity_bruteforce(const float* vertices, const uint32_t* tris, uint64_t n_tris, const float* points,
+                               uint64_t n_points, uint8_t* parity_out /* 3 per point */, int n_threads);
+
+/* mesh_check.rs:24-81 (returns 0 when Ok, else an error kind 1..5) */
+int orc_check_mesh_triangle_list(const float* vertices, uint64_t n_vertices, const uint32_t* faces,
+                                 uint64_t n_faces_idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,175 @@
+"""GPU (through the C ABI) against the oracle on the same seeded inputs.
+
+Point queries: bit-exact (the device evaluates the reference's FP32
+expressions in the reference's order, so every distance and sign matches).
+Grid: bit-exact against the oracle's restatement of the device's frontier
+schedule; against the reference's own heap schedule the signs are bit-exact
+and the distances are measured (the reference's flood fill is itself
+schedule-dependent: its 1-thread and 8-thread runs disagree on ~2% of cells).
+"""
+import numpy as np
+import pytest
+
+from mesh_to_sdf_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def assert_bit_exact(gpu, ref, what):
+    assert gpu.shape == ref.shape, what
+    bad = np.nonzero(bits(gpu) != bits(ref))[0]
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first {bad[:5]} gpu={gpu[bad[:5]]} ref={ref[bad[:5]]}"
+
+
+METHODS = [(0, 0), (0, 1), (1, 0), (1, 1), (2, 1), (3, 0)]
+
+
+def _accel(sdf, accel, sign):
+    M = sdf.AccelerationMethod
+    return [M.None_(sign), M.Bvh(sign), M.Rtree, M.RtreeBvh][accel]
+
+
+def _meshes():
+    v2, f2 = S.icosphere(2)
+    vt, ft = S.torus(1.0, 0.35, 32, 16, 0.002)
+    vn, fn = S.noise_sphere(3, 0.05)
+    return {"icosphere2": (v2, f2), "torus32x16": (vt, ft), "noise_sphere3": (vn, fn)}
+
+
+@pytest.mark.parametrize("mesh", ["icosphere2", "torus32x16", "noise_sphere3"])
+@pytest.mark.parametrize("accel,sign", METHODS)
+def test_point_methods_bit_exact(sdf, oracle, mesh, accel, sign):
+    v, f = _meshes()[mesh]
+    diag = S.bbox_diagonal(v)
+    q = np.concatenate([S.uniform_queries(v, 3000, 101), S.surface_queries(v, f, 3000, 0.01 * diag, 102)])
+    ref_s, ref = oracle.generate_sdf(v, f, oracle.TOPO_LIST, q, accel, sign)
+    assert ref_s == 0
+    gpu = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, _accel(sdf, accel, sign))
+    assert_bit_exact(gpu, ref, f"{mesh} accel={accel} sign={sign}")
+
+
+def test_suzanne_all_methods_bit_exact(sdf, oracle, suzanne):
+    v, idx = suzanne
+    f = idx.reshape(-1, 3)
+    q = np.concatenate([S.uniform_queries(v, 2000, 103), S.surface_queries(v, f, 2000, 0.02, 104)])
+    for accel, sign in METHODS:
+        s, ref = oracle.generate_sdf(v, idx, oracle.TOPO_LIST, q, accel, sign)
+        gpu = sdf.generate_sdf(v, sdf.Topology.TriangleList(idx), q, _accel(sdf, accel, sign))
+        assert_bit_exact(gpu, ref, f"suzanne accel={accel} sign={sign}")
+
+
+@pytest.mark.parametrize("accel,sign", METHODS)
+def test_topologies(sdf, oracle, accel, sign):
+    """Strips and index-free lists decode like lib.rs:193-215."""
+    g = np.random.Generator(np.random.PCG64(5))
+    v = g.uniform(-1, 1, size=(40, 3)).astype(np.float32)
+    strip = g.integers(0, 40, size=30).astype(np.uint32)
+    q = g.uniform(-1.5, 1.5, size=(500, 3)).astype(np.float32)
+    for topo, idx in ((oracle.TOPO_STRIP, strip), (oracle.TOPO_STRIP, None), (oracle.TOPO_LIST, None)):
+        vv = v[:39] if (topo == oracle.TOPO_LIST and idx is None) else v
+        s, ref = oracle.generate_sdf(vv, idx, topo, q, accel, sign)
+        T = sdf.Topology.TriangleStrip(idx) if topo == oracle.TOPO_STRIP else sdf.Topology.TriangleList(idx)
+        gpu = sdf.generate_sdf(vv, T, q, _accel(sdf, accel, sign))
+        assert_bit_exact(gpu, ref, f"topo={topo} idx={'yes' if idx is not None else 'no'}")
+
+
+def test_config1_rtree_bvh_full(sdf, oracle):
+    """Unit icosphere L3, 10,000 uniform queries, RtreeBvh: the full workload."""
+    v, f, q = S.config1()
+    s, ref = oracle.generate_sdf(v, f, oracle.TOPO_LIST, q, oracle.ACCEL_RTREE_BVH, oracle.SIGN_RAYCAST)
+    gpu = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, sdf.AccelerationMethod.RtreeBvh)
+    assert_bit_exact(gpu, ref, "config1")
+
+
+def test_config3_shape_reduced(sdf, oracle):
+    """Noise sphere + half near-surface / half uniform queries, Bvh(Raycast), at
+    level 5 (20,480 triangles) and 100k queries (the oracle's BVH keeps it fast)."""
+    v, f, q = S.config3(n_queries=100_000, level=5)
+    s, ref = oracle.generate_sdf(v, f, oracle.TOPO_LIST, q, oracle.ACCEL_BVH, oracle.SIGN_RAYCAST)
+    gpu = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast))
+    assert_bit_exact(gpu, ref, "config3 reduced")
+    diag = S.bbox_diagonal(v)
+    d64 = oracle.fp64_distance(v, f, q[:2000])
+    err = np.abs(np.abs(gpu[:2000]).astype(np.float64) - d64) / diag
+    assert err.max() < 1e-6
+
+
+def test_resident_mesh_matches_generate(sdf, oracle):
+    """query_sdf(build(..)) is identical to generate_sdf (e.g. generic/rtree_bvh.rs:331-334)."""
+    v, f = S.noise_sphere(3, 0.05)
+    q = S.uniform_queries(v, 5000, 105)
+    topo = sdf.Topology.TriangleList(f)
+    for accel, sign in METHODS:
+        m = _accel(sdf, accel, sign)
+        acc = sdf.build_sdf_acceleration(v, topo, m)
+        a = sdf.query_sdf(acc, q)
+        b = sdf.generate_sdf(v, topo, q, m)
+        assert_bit_exact(a, b, f"resident {m}")
+        acc.close()
+
+
+def test_closest_vectors_bit_exact(sdf, oracle):
+    v, f = S.torus(1.0, 0.35, 32, 16, 0.002)
+    q = S.uniform_queries(v, 4000, 106)
+    s, vec_ref, in_ref = oracle.query_vector_to_closest_point(v, f, oracle.TOPO_LIST, q)
+    acc = sdf.build_sdf_acceleration(v, sdf.Topology.TriangleList(f), sdf.AccelerationMethod.RtreeBvh)
+    vec, inside = sdf.query_vector_to_closest_point_rtree_bvh(acc, q)
+    assert np.array_equal(vec.view(np.uint32), vec_ref.view(np.uint32))
+    assert np.array_equal(inside, in_ref)
+
+
+# ------------------------------------------------------------------ grid
+
+@pytest.mark.parametrize("n", [16, 48])
+@pytest.mark.parametrize("sign", [0, 1])
+def test_grid_bit_exact_vs_frontier_schedule(sdf, oracle, n, sign):
+    v, f = S.torus(1.0, 0.35, 64, 32, 0.002)
+    first, size, count = S.grid_for(v, (n, n + 3, n - 2))
+    s, ref, st = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, first, size, count, sign, 8, oracle.GRID_FRONTIER)
+    assert s == 0
+    grid = sdf.Grid(first, size, count)
+    gpu = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), grid, sign)
+    assert_bit_exact(gpu, ref, f"grid n={n} sign={sign}")
+
+
+def test_grid_config2_full(sdf, oracle):
+    """Jittered torus (65,536 triangles), 128^3 grid, Raycast: bit-exact against
+    the frontier schedule; signs bit-exact against the reference's 1-thread
+    heap schedule; distance differences against it measured and bounded."""
+    v, f, first, size, count = S.config2()
+    grid = sdf.Grid(first, size, count)
+    gpu = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), grid, sdf.SignMethod.Raycast)
+    s, ref, _ = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, first, size, count, 0, 16, oracle.GRID_FRONTIER)
+    assert_bit_exact(gpu, ref, "config2 frontier")
+    s, heap, _ = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, first, size, count, 0, 1, oracle.GRID_SEQUENTIAL)
+    assert np.array_equal(np.signbit(gpu), np.signbit(heap))
+    diff = np.abs(gpu - heap)
+    frac = float(np.mean(diff > 0))
+    print(f"config2: cells differing from the 1-thread heap schedule: {frac:.4%}, max |diff| {diff.max():.3g}")
+    assert frac < 0.08 and diff.max() < 0.01
+
+
+def test_grid_deterministic(sdf):
+    v, f = S.torus(1.0, 0.35, 64, 32, 0.002)
+    first, size, count = S.grid_for(v, (40, 40, 40))
+    grid = sdf.Grid(first, size, count)
+    a = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), grid, 0)
+    b = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), grid, 0)
+    assert np.array_equal(bits(a), bits(b))
+
+
+def test_grid_partial_and_negative_size(sdf, oracle):
+    """A grid covering part of the mesh (generate/grid.rs:808-840) and a grid with
+    a negative cell size (grid.rs:179: sizes may be negative)."""
+    v, f = S.icosphere(3)
+    lo, hi = S.aabb(v)
+    for first, size in (oracle.grid_from_bounding_box(lo, hi * np.float32(0.5), [20, 20, 20]),
+                        (np.array([1.2, -1.2, 1.2], np.float32), np.array([-0.1, 0.1, -0.1], np.float32))):
+        s, ref, _ = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, first, size, [20, 20, 20], 0, 8,
+                                             oracle.GRID_FRONTIER)
+        gpu = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), sdf.Grid(first, size, [20, 20, 20]), 0)
+        assert_bit_exact(gpu, ref, f"grid size={size}")
