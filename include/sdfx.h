@@ -143,6 +143,14 @@ sdfx_status sdfx_query_vector_to_closest_point(const sdfx_mesh* mesh, const floa
 /* Drops an SdfAccelerationMesh (lib.rs:303-312 is Clone + Drop). NULL is allowed. */
 void sdfx_free_sdf_acceleration(sdfx_mesh* mesh);
 
+/* ---- Measurement (no counterpart in the reference; for benchmark and
+ * profiling tools). While enabled, every kernel launch is bracketed by a HIP
+ * event pair on its stream. sdfx_profile_enable(on) also clears the totals.
+ * sdfx_profile_read fills up to max_entries rows (name: 64 bytes each, total
+ * milliseconds, launch count) and returns the number of kernel names seen. */
+void sdfx_profile_enable(int on);
+int sdfx_profile_read(int max_entries, char* names, double* total_ms, uint64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,8 @@ SIGNATURES = {
     "sdfx_query_sdf": (_i, [_vp, _f, _u64v, _f, _u64]),
     "sdfx_query_vector_to_closest_point": (_i, [_vp, _f, _u64v, _f, _u8]),
     "sdfx_free_sdf_acceleration": (None, [_vp]),
+    "sdfx_profile_enable": (None, [_i]),
+    "sdfx_profile_read": (_i, [_i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _u64]),
 }
 
 _lib = None
@@ -78,6 +80,23 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def profile_enable(on: bool = True) -> None:
+    lib().sdfx_profile_enable(1 if on else 0)
+
+
+def profile_read(max_entries: int = 128) -> dict:
+    """{kernel name: (total ms, launches)} since the last profile_enable()."""
+    names = ctypes.create_string_buffer(64 * max_entries)
+    ms = (ctypes.c_double * max_entries)()
+    cnt = (ctypes.c_uint64 * max_entries)()
+    n = lib().sdfx_profile_read(max_entries, names, ms, cnt)
+    out = {}
+    for i in range(min(n, max_entries)):
+        name = names.raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
+        out[name] = (ms[i], int(cnt[i]))
+    return out
 
 
 def check(status: int) -> int:

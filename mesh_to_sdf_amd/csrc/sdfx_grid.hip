@@ -392,15 +392,15 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     hipStream_t s = A.stream;
     const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
     uint64_t* keys = A.alloc<uint64_t>(ncells);
-    hipLaunchKernelGGL(k_fill_u64, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, KEY_EMPTY);
+    SDFX_LAUNCH(k_fill_u64, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, KEY_EMPTY);
 
     // 1. seeding
     SeedBox* boxes = A.alloc<SeedBox>(nt);
     uint64_t* counts = A.alloc<uint64_t>(nt);
     uint64_t* total = A.alloc<uint64_t>(1);
-    hipLaunchKernelGGL(k_seed_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, g, boxes, counts);
+    SDFX_LAUNCH(k_seed_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, g, boxes, counts);
     exclusive_scan_u64(counts, counts, nt, total, A);
-    hipLaunchKernelGGL(k_seed_pairs<SIGN>, dim3(8192), dim3(256), 0, s, d_tris, nt, g, boxes, counts, total, keys,
+    SDFX_LAUNCH(k_seed_pairs<SIGN>, dim3(8192), dim3(256), 0, s, d_tris, nt, g, boxes, counts, total, keys,
                        d_flags);
     SDFX_HIP(hipGetLastError());
 
@@ -409,10 +409,10 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     uint32_t* cnt = A.alloc<uint32_t>(2);
     uint8_t* changed = A.alloc<uint8_t>((ncells + 3) & ~3ull);
     FrontierItem* items = A.alloc<FrontierItem>(ncells);
-    hipLaunchKernelGGL(k_fill_u8, dim3(stride_blocks(ncells)), dim3(256), 0, s, changed, (ncells + 3) & ~3ull,
+    SDFX_LAUNCH(k_fill_u8, dim3(stride_blocks(ncells)), dim3(256), 0, s, changed, (ncells + 3) & ~3ull,
                        (uint8_t)0);
-    hipLaunchKernelGGL(k_fill_u32g, dim3(1), dim3(64), 0, s, cnt, 2ull, 0u);
-    hipLaunchKernelGGL(k_frontier_init, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, lists[0], cnt);
+    SDFX_LAUNCH(k_fill_u32g, dim3(1), dim3(64), 0, s, cnt, 2ull, 0u);
+    SDFX_LAUNCH(k_frontier_init, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, lists[0], cnt);
     SDFX_HIP(hipGetLastError());
     uint32_t* h_cnt = pinned_u32();
     const unsigned prop_blocks = 4096;
@@ -426,9 +426,9 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
             if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
             for (int b = 0; b < batch; ++b, ++round) {
                 int nxt = cur ^ 1;
-                hipLaunchKernelGGL(k_frontier_snapshot, dim3(2048), dim3(256), 0, s, lists[cur], cnt + cur,
+                SDFX_LAUNCH(k_frontier_snapshot, dim3(2048), dim3(256), 0, s, lists[cur], cnt + cur,
                                    cnt + nxt, keys, changed, items, (int)SIGN);
-                hipLaunchKernelGGL(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items,
+                SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items,
                                    cnt + cur, d_tris, g, keys, changed, lists[nxt], cnt + nxt, d_flags);
                 cur = nxt;
             }
@@ -447,18 +447,18 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
         for (int a = 0; a < 3; ++a) {
             uint64_t nw = rows[a] * g.words[a];
             bufs[a] = A.alloc<uint32_t>(nw);
-            hipLaunchKernelGGL(k_fill_u32g, dim3(stride_blocks(nw)), dim3(256), 0, s, bufs[a], nw, 0u);
+            SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(nw)), dim3(256), 0, s, bufs[a], nw, 0u);
         }
         bx = bufs[0];
         by = bufs[1];
         bz = bufs[2];
-        hipLaunchKernelGGL(k_ray_scatter, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, d_tris, nt, g, bx, by, bz);
+        SDFX_LAUNCH(k_ray_scatter, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, d_tris, nt, g, bx, by, bz);
         for (int a = 0; a < 3; ++a)
-            hipLaunchKernelGGL(k_ray_suffix, dim3(blocks_for(rows[a], 256)), dim3(256), 0, s, bufs[a], rows[a],
+            SDFX_LAUNCH(k_ray_suffix, dim3(blocks_for(rows[a], 256)), dim3(256), 0, s, bufs[a], rows[a],
                                g.words[a]);
         SDFX_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_grid_finish<SIGN>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, bx, by, bz,
+    SDFX_LAUNCH(k_grid_finish<SIGN>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, bx, by, bz,
                        d_out);
     SDFX_HIP(hipGetLastError());
 }
@@ -476,7 +476,7 @@ void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign
     }
     if (nt == 0) {
         // no triangle: every cell keeps f32::MAX (grid.rs:137-143), no ray hits
-        hipLaunchKernelGGL(k_fill_u32g, dim3(stride_blocks(ncells)), dim3(256), 0, A.stream, (uint32_t*)d_out, ncells,
+        SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(ncells)), dim3(256), 0, A.stream, (uint32_t*)d_out, ncells,
                            fbits(FLT_MAX));
         SDFX_HIP(hipGetLastError());
         return;

@@ -187,9 +187,9 @@ static void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bit
     int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, A.stream, ki, n, shift, hist, nblocks);
+        SDFX_LAUNCH(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, A.stream, ki, n, shift, hist, nblocks);
         exclusive_scan_u32(hist, hist, (uint64_t)256 * nblocks, nullptr, A);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_THREADS), 0, A.stream, ki, vi, n, shift, hist,
+        SDFX_LAUNCH(k_rs_scatter, dim3(nblocks), dim3(RS_THREADS), 0, A.stream, ki, vi, n, shift, hist,
                            nblocks, ko, vo);
         SDFX_HIP(hipGetLastError());
         uint32_t* t;
@@ -324,13 +324,13 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
     uint32_t* keys = A.alloc<uint32_t>(n);
     uint32_t* vals = A.alloc<uint32_t>(n);
     Box* leaf_box = A.alloc<Box>(n);
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, bounds);
+    SDFX_LAUNCH(k_init_bounds, dim3(1), dim3(64), 0, s, bounds);
     unsigned bb = blocks_for(n, 256);
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(bb > 1024 ? 1024 : bb), dim3(256), 0, s, d_tris, n, bounds);
-    hipLaunchKernelGGL(k_morton, dim3(bb), dim3(256), 0, s, d_tris, n, bounds, keys, vals);
+    SDFX_LAUNCH(k_centroid_bounds, dim3(bb > 1024 ? 1024 : bb), dim3(256), 0, s, d_tris, n, bounds);
+    SDFX_LAUNCH(k_morton, dim3(bb), dim3(256), 0, s, d_tris, n, bounds, keys, vals);
     SDFX_HIP(hipGetLastError());
     radix_sort_pairs(keys, vals, n, 30, A);
-    hipLaunchKernelGGL(k_leaves, dim3(bb), dim3(256), 0, s, d_tris, vals, n, bvh.leaves.get(), leaf_box);
+    SDFX_LAUNCH(k_leaves, dim3(bb), dim3(256), 0, s, d_tris, vals, n, bvh.leaves.get(), leaf_box);
     SDFX_HIP(hipGetLastError());
     if (n == 1) {
         bvh.root = ~0;
@@ -344,10 +344,10 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
     int32_t* parent_leaf = A.alloc<int32_t>(n);
     Box* node_box = A.alloc<Box>(ni);
     uint32_t* visit = A.alloc<uint32_t>(ni);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(ni, 256)), dim3(256), 0, s, visit, ni);
-    hipLaunchKernelGGL(k_karras, dim3(blocks_for(ni, 256)), dim3(256), 0, s, keys, (int64_t)n, left, right,
+    SDFX_LAUNCH(k_zero_u32, dim3(blocks_for(ni, 256)), dim3(256), 0, s, visit, ni);
+    SDFX_LAUNCH(k_karras, dim3(blocks_for(ni, 256)), dim3(256), 0, s, keys, (int64_t)n, left, right,
                        parent_int, parent_leaf);
-    hipLaunchKernelGGL(k_refit, dim3(bb), dim3(256), 0, s, n, parent_leaf, parent_int, left, right, leaf_box,
+    SDFX_LAUNCH(k_refit, dim3(bb), dim3(256), 0, s, n, parent_leaf, parent_int, left, right, leaf_box,
                        node_box, visit, bvh.nodes.get());
     SDFX_HIP(hipGetLastError());
 }
@@ -630,19 +630,19 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     unsigned blocks = blocks_for(nq, 256);
     switch (mode) {
         case QM_RAYCAST_VOTE:
-            hipLaunchKernelGGL(k_lbvh_query<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+            SDFX_LAUNCH(k_lbvh_query<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
                                bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
             break;
         case QM_NORMAL_FOLD:
-            hipLaunchKernelGGL(k_lbvh_query<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+            SDFX_LAUNCH(k_lbvh_query<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
                                bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
             break;
         case QM_NEAREST_SIGNED:
-            hipLaunchKernelGGL(k_lbvh_query<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+            SDFX_LAUNCH(k_lbvh_query<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
                                bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
             break;
         case QM_NEAREST_VOTE:
-            hipLaunchKernelGGL(k_lbvh_query<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
+            SDFX_LAUNCH(k_lbvh_query<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
                                bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
             break;
     }
@@ -677,7 +677,7 @@ __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restri
 void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float* d_vec, uint8_t* d_inside,
                           DevFlags* d_flags, hipStream_t s) {
     if (!nq || !bvh.n) return;
-    hipLaunchKernelGGL(k_closest_vectors, dim3(blocks_for(nq, 256)), dim3(256), 0, s, bvh.nodes.get(),
+    SDFX_LAUNCH(k_closest_vectors, dim3(blocks_for(nq, 256)), dim3(256), 0, s, bvh.nodes.get(),
                        bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_vec, d_inside, d_flags);
     SDFX_HIP(hipGetLastError());
 }

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) k_gather(const float* __restrict__ v, uin
 void gather_triangles(const float* d_v, uint64_t nv, const uint32_t* d_idx, sdfx_topology topo, uint64_t nt,
                       TriRec* d_tris, DevFlags* d_flags, hipStream_t s) {
     if (!nt) return;
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_v, nv, d_idx, (int)topo, nt, d_tris,
+    SDFX_LAUNCH(k_gather, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_v, nv, d_idx, (int)topo, nt, d_tris,
                        d_flags);
     SDFX_HIP(hipGetLastError());
 }
@@ -77,7 +77,7 @@ void check_queries_finite(const float* d_q, uint64_t nq, DevFlags* d_flags, hipS
     uint64_t n = 3 * nq;
     unsigned blocks = blocks_for(n, 256);
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_check_finite, dim3(blocks), dim3(256), 0, s, d_q, n, d_flags);
+    SDFX_LAUNCH(k_check_finite, dim3(blocks), dim3(256), 0, s, d_q, n, d_flags);
     SDFX_HIP(hipGetLastError());
 }
 
@@ -90,7 +90,7 @@ void fill_u32(uint32_t* d_ptr, uint64_t n, uint32_t v, hipStream_t s) {
     if (!n) return;
     unsigned blocks = blocks_for(n, 256);
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, d_ptr, n, v);
+    SDFX_LAUNCH(k_fill, dim3(blocks), dim3(256), 0, s, d_ptr, n, v);
     SDFX_HIP(hipGetLastError());
 }
 
@@ -156,15 +156,15 @@ template <typename T>
 static void scan_rec(const T* d_in, T* d_out, uint64_t n, T* d_total, Arena& A) {
     uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     T* sums = A.alloc<T>(tiles);
-    hipLaunchKernelGGL(k_scan_tile<T>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, A.stream, d_in, d_out, n, sums);
+    SDFX_LAUNCH(k_scan_tile<T>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, A.stream, d_in, d_out, n, sums);
     SDFX_HIP(hipGetLastError());
     if (tiles == 1) {
-        if (d_total) hipLaunchKernelGGL(k_copy_one<T>, dim3(1), dim3(1), 0, A.stream, sums, d_total);
+        if (d_total) SDFX_LAUNCH(k_copy_one<T>, dim3(1), dim3(1), 0, A.stream, sums, d_total);
         return;
     }
     T* offs = A.alloc<T>(tiles);
     scan_rec<T>(sums, offs, tiles, d_total, A);
-    hipLaunchKernelGGL(k_scan_add<T>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, A.stream, d_out, n, offs);
+    SDFX_LAUNCH(k_scan_add<T>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, A.stream, d_out, n, offs);
     SDFX_HIP(hipGetLastError());
 }
 

@@ -99,7 +99,7 @@ void none_query(const TriRec* d_tris, uint64_t nt, const float* d_q, uint64_t nq
     hipStream_t s = A.stream;
     unsigned qblocks = blocks_for(nq, NONE_THREADS);
     if (sign == SDFX_SIGN_NORMAL) {
-        hipLaunchKernelGGL(k_none_normal, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, nt, d_q, nq, d_out,
+        SDFX_LAUNCH(k_none_normal, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, nt, d_q, nq, d_out,
                            d_flags);
         SDFX_HIP(hipGetLastError());
         return;
@@ -114,7 +114,7 @@ void none_query(const TriRec* d_tris, uint64_t nt, const float* d_q, uint64_t nq
         if (chunks > 65535) chunks = 65535;
     }
     if (chunks <= 1) {
-        hipLaunchKernelGGL(k_none_raycast<false>, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, nt, nt, d_q, nq,
+        SDFX_LAUNCH(k_none_raycast<false>, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, nt, nt, d_q, nq,
                            d_out, nullptr, nullptr);
         SDFX_HIP(hipGetLastError());
         return;
@@ -123,11 +123,11 @@ void none_query(const TriRec* d_tris, uint64_t nt, const float* d_q, uint64_t nq
     chunks = (nt + chunk - 1) / chunk;
     uint32_t* best = A.alloc<uint32_t>(nq);
     uint32_t* counts = A.alloc<uint32_t>(nq);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nq, 256)), dim3(256), 0, s, best, nq, 0x7f800000u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nq, 256)), dim3(256), 0, s, counts, nq, 0u);
-    hipLaunchKernelGGL(k_none_raycast<true>, dim3(qblocks, (unsigned)chunks), dim3(NONE_THREADS), 0, s, d_tris, nt,
+    SDFX_LAUNCH(k_fill_u32, dim3(blocks_for(nq, 256)), dim3(256), 0, s, best, nq, 0x7f800000u);
+    SDFX_LAUNCH(k_fill_u32, dim3(blocks_for(nq, 256)), dim3(256), 0, s, counts, nq, 0u);
+    SDFX_LAUNCH(k_none_raycast<true>, dim3(qblocks, (unsigned)chunks), dim3(NONE_THREADS), 0, s, d_tris, nt,
                        chunk, d_q, nq, d_out, best, counts);
-    hipLaunchKernelGGL(k_none_raycast_finish, dim3(qblocks), dim3(NONE_THREADS), 0, s, best, counts, nq, d_out);
+    SDFX_LAUNCH(k_none_raycast_finish, dim3(qblocks), dim3(NONE_THREADS), 0, s, best, counts, nq, d_out);
     SDFX_HIP(hipGetLastError());
 }
 

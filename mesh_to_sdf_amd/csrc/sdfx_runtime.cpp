@@ -1,6 +1,7 @@
 // sdfx_runtime.cpp — error state, per-thread streams, caching allocator.
 #include "sdfx_runtime.hpp"
 
+#include <cstdio>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -111,4 +112,94 @@ void dev_free(void* p) {
     P.live.erase(it);
 }
 
+// ---------------------------------------------------------------- profiling
+
+namespace {
+struct ProfRecord {
+    std::string name;
+    hipEvent_t a, b;
+};
+struct ProfState {
+    std::mutex mu;
+    bool enabled = false;
+    std::vector<ProfRecord> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, std::pair<double, uint64_t>> totals;  // name -> (ms, launches)
+};
+ProfState& prof() {
+    static ProfState* p = new ProfState();
+    return *p;
+}
+hipEvent_t take_event(ProfState& P) {
+    if (!P.free_events.empty()) {
+        hipEvent_t e = P.free_events.back();
+        P.free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+void resolve(ProfState& P) {
+    for (auto& r : P.pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& t = P.totals[r.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        P.free_events.push_back(r.a);
+        P.free_events.push_back(r.b);
+    }
+    P.pending.clear();
+}
+}  // namespace
+
+bool profile_enabled() { return prof().enabled; }
+
+ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+    ProfState& P = prof();
+    if (!P.enabled) return;
+    std::lock_guard<std::mutex> lk(P.mu);
+    start = take_event(P);
+    if (start) (void)hipEventRecord(start, stream);
+}
+
+ProfScope::~ProfScope() {
+    if (!start) return;
+    ProfState& P = prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipEvent_t stop = take_event(P);
+    if (!stop) return;
+    (void)hipEventRecord(stop, stream);
+    P.pending.push_back({name, start, stop});
+}
+
 }  // namespace sdfx
+
+extern "C" {
+
+void sdfx_profile_enable(int on) {
+    auto& P = sdfx::prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    sdfx::resolve(P);
+    P.totals.clear();
+    P.enabled = on != 0;
+}
+
+int sdfx_profile_read(int max_entries, char* names, double* total_ms, uint64_t* launches) {
+    auto& P = sdfx::prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    sdfx::resolve(P);
+    int i = 0;
+    for (auto& kv : P.totals) {
+        if (i >= max_entries) break;
+        std::snprintf(names + 64 * i, 64, "%s", kv.first.c_str());
+        total_ms[i] = kv.second.first;
+        launches[i] = kv.second.second;
+        ++i;
+    }
+    return (int)P.totals.size();
+}
+
+}  // extern "C"

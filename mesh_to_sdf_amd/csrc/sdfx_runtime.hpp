@@ -111,6 +111,27 @@ struct DevFlags {
 
 int current_device();
 
+// Kernel timing for measurement tools (off by default). While enabled, each
+// ProfScope records a HIP event pair around the launches it encloses on the
+// given stream; profile_read() resolves them into per-name totals.
+bool profile_enabled();
+struct ProfScope {
+    ProfScope(const char* name, hipStream_t stream);
+    ~ProfScope();
+    ProfScope(const ProfScope&) = delete;
+    ProfScope& operator=(const ProfScope&) = delete;
+    const char* name;
+    hipStream_t stream;
+    hipEvent_t start = nullptr;
+};
+
+// hipLaunchKernelGGL with an optional timing scope named after the kernel.
+#define SDFX_LAUNCH(kernel, grid, block, shmem, stream, ...)                         \
+    do {                                                                             \
+        ::sdfx::ProfScope sdfx_ps_(#kernel, stream);                                 \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);         \
+    } while (0)
+
 // Launch helper: blocks for n items with `threads` per block.
 inline unsigned blocks_for(uint64_t n, unsigned threads) {
     uint64_t b = (n + threads - 1) / threads;
