@@ -101,14 +101,61 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+
+    def timed(k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    elapsed = timed(args.steps)
     ms = elapsed * 1000.0 / args.steps
     value = samples * args.steps / elapsed
+
+    # Kernel timing: the same K steps again with HIP events around every launch
+    # (on the launch stream), plus exact work counters.
+    _lib.profile_enable(True)
+    timed(args.steps)
+    prof = _lib.profile_read()
+    counters = {k: _lib.profile_counter(k) for k in ("grid.seed_distance_evals", "grid.propagate_distance_evals")}
+    _lib.profile_enable(False)
+    kernels = sorted(((name, t, n) for name, (t, n) in prof.items()), key=lambda x: -x[1])
+    dom_name, dom_ms, dom_launches = kernels[0]
+    mesh_bytes = 12 * len(v) + 12 * len(f)
+    if args.workload == "grid":
+        # 80 FP32 operations per point-triangle distance (geo.rs:26-30, 70-138 as written)
+        evals = None
+        if dom_name.startswith("k_frontier_propagate"):
+            evals = counters["grid.propagate_distance_evals"]
+        elif dom_name.startswith("k_seed_pairs"):
+            evals = counters["grid.seed_distance_evals"]
+        if evals is not None:
+            flops_per_launch = 80.0 * evals / dom_launches
+            achieved = flops_per_launch / (dom_ms / dom_launches / 1e3) / 1e12
+            roofline = {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": achieved / 157.3, "traffic": None, "kernel": dom_name,
+                        "engine": "FP32 VALU (no MFMA: 157.3 TFLOP/s is MI355X's FP32 vector peak)",
+                        "work_per_launch": f"{evals / dom_launches:.0f} distance evaluations x 80 FLOP",
+                        "avg_launch_ms": dom_ms / dom_launches}
+        else:
+            achieved = (4.0 * samples + mesh_bytes) / (dom_ms / dom_launches / 1e3) / 1e9
+            roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                        "frac": achieved / 8000.0, "traffic": None, "kernel": dom_name}
+        step_bytes = 4.0 * samples + mesh_bytes
+        roofline_hbm_step = {"bytes_per_step": step_bytes, "GB/s": step_bytes / (ms / 1e3) / 1e9,
+                             "frac": step_bytes / (ms / 1e3) / 1e9 / 8000.0,
+                             "per_sample_bytes": step_bytes / samples}
+    else:
+        per_launch = 16.0 * samples + mesh_bytes  # 12 B query + 4 B result per sample, mesh once
+        achieved = per_launch / (dom_ms / dom_launches / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                    "frac": achieved / 8000.0, "traffic": None, "kernel": dom_name,
+                    "avg_launch_ms": dom_ms / dom_launches}
+        roofline_hbm_step = None
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, v, f, locals())
 
     result = {
         "metric": "SDF samples/sec (1 GPU)",
@@ -124,10 +171,63 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded mesh generators)",
         "config": config,
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_ms_per_step": {n: round(t / args.steps, 4) for n, t, _ in kernels[:8]},
     }
+    if roofline_hbm_step:
+        result["roofline_hbm_step"] = roofline_hbm_step
+    if cpu and cpu.get("speedup") is None:
+        cpu["speedup"] = value / cpu["value"] if cpu["value"] else None
     print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(workload, v, f, env):
+    """The oracle's restatement of the reference's CPU algorithm on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    from mesh_to_sdf_amd import synthetic as S
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    budget_s, times = 30.0, []
+    if workload == "grid":
+        first, size, count = env["first"], env["size"], env["count"]
+        samples = count[0] * count[1] * count[2]
+        t_start = time.perf_counter()
+        while len(times) < 3 and time.perf_counter() - t_start < budget_s:
+            t0 = time.perf_counter()
+            s, out, _ = O.generate_grid_sdf(v, f, O.TOPO_LIST, first, size, count, O.SIGN_RAYCAST, threads,
+                                            O.GRID_THREADED)
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[len(times) // 2]
+        gpu = env["dout"].cpu().numpy()
+        diag = S.bbox_diagonal(v)
+        diff = np.abs(gpu - out)
+        accuracy = {"sign_mismatches_vs_reference_schedule": int(np.sum(np.signbit(gpu) != np.signbit(out))),
+                    "cells_differing": int(np.sum(diff > 0)), "max_abs_diff_over_diag": float(diff.max() / diag)}
+        return {"value": samples / t, "unit": "samples/s", "cores": threads, "kind": "port",
+                "sample": f"full workload ({len(f)} tris, {samples} cells); the oracle's restatement of "
+                          f"generate/grid.rs with the heap split over {threads} threads; median of {len(times)}",
+                "seconds": t, "gpu_vs_this_run": accuracy, "speedup": None}
+    if workload == "rtree_bvh":
+        q = env["q"]
+        t_start = time.perf_counter()
+        while len(times) < 5 and time.perf_counter() - t_start < budget_s:
+            t0 = time.perf_counter()
+            O.generate_sdf(v, f, O.TOPO_LIST, q, O.ACCEL_RTREE_BVH, O.SIGN_RAYCAST, 1)
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[len(times) // 2]
+        return {"value": len(q) / t, "unit": "samples/s", "cores": 1, "kind": "port",
+                "sample": "full workload; single-threaded like generic/rtree_bvh.rs:150-152", "seconds": t,
+                "speedup": None}
+    q = env["q"][:200_000]
+    t0 = time.perf_counter()
+    O.generate_sdf(v, f, O.TOPO_LIST, q, O.ACCEL_BVH, O.SIGN_RAYCAST, threads)
+    t = time.perf_counter() - t0
+    return {"value": len(q) / t, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"first 200,000 of the 4,194,304 queries, {threads} threads (rayon par_iter in the reference)",
+            "seconds": t, "speedup": None}
 
 
 if __name__ == "__main__":

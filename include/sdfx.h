@@ -143,6 +143,14 @@ sdfx_status sdfx_query_vector_to_closest_point(const sdfx_mesh* mesh, const floa
 /* Drops an SdfAccelerationMesh (lib.rs:303-312 is Clone + Drop). NULL is allowed. */
 void sdfx_free_sdf_acceleration(sdfx_mesh* mesh);
 
+/* ---- Tuning (no counterpart in the reference). Process-wide knobs; returns 0
+ * on success, -1 for an unknown key.
+ *   "exhaustive_max_pairs": queries x triangles at or below which Bvh, Rtree
+ *   and RtreeBvh evaluate every triangle instead of building the LBVH (results
+ *   are identical either way); <= 0 always builds the LBVH. */
+int sdfx_set_tuning(const char* key, int64_t value);
+int64_t sdfx_get_tuning(const char* key);
+
 /* ---- Measurement (no counterpart in the reference; for benchmark and
  * profiling tools). While enabled, every kernel launch is bracketed by a HIP
  * event pair on its stream. sdfx_profile_enable(on) also clears the totals.
@@ -150,6 +158,8 @@ void sdfx_free_sdf_acceleration(sdfx_mesh* mesh);
  * milliseconds, launch count) and returns the number of kernel names seen. */
 void sdfx_profile_enable(int on);
 int sdfx_profile_read(int max_entries, char* names, double* total_ms, uint64_t* launches);
+/* Work counters accumulated while profiling (e.g. "grid.propagate_distance_evals"). */
+uint64_t sdfx_profile_counter(const char* name);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,9 @@ SIGNATURES = {
     "sdfx_free_sdf_acceleration": (None, [_vp]),
     "sdfx_profile_enable": (None, [_i]),
     "sdfx_profile_read": (_i, [_i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _u64]),
+    "sdfx_profile_counter": (ctypes.c_uint64, [ctypes.c_char_p]),
+    "sdfx_set_tuning": (_i, [ctypes.c_char_p, ctypes.c_int64]),
+    "sdfx_get_tuning": (ctypes.c_int64, [ctypes.c_char_p]),
 }
 
 _lib = None
@@ -97,6 +100,33 @@ def profile_read(max_entries: int = 128) -> dict:
         name = names.raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
         out[name] = (ms[i], int(cnt[i]))
     return out
+
+
+def profile_counter(name: str) -> int:
+    return int(lib().sdfx_profile_counter(name.encode()))
+
+
+def set_tuning(key: str, value: int) -> None:
+    if lib().sdfx_set_tuning(key.encode(), int(value)) != 0:
+        raise KeyError(key)
+
+
+def get_tuning(key: str) -> int:
+    return int(lib().sdfx_get_tuning(key.encode()))
+
+
+class tuned:
+    """Context manager setting a tuning knob and restoring it on exit."""
+
+    def __init__(self, key: str, value: int):
+        self.key, self.value = key, value
+
+    def __enter__(self):
+        self.old = get_tuning(self.key)
+        set_tuning(self.key, self.value)
+
+    def __exit__(self, *a):
+        set_tuning(self.key, self.old)
 
 
 def check(status: int) -> int:

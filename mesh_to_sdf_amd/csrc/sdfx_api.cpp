@@ -114,6 +114,12 @@ void check_internal(const DevFlags& f) {
 }
 [[noreturn]] void fail_empty_mesh() { fail(SDFX_ERR_EMPTY_MESH, "called `Option::unwrap()` on a `None` value"); }
 
+bool use_exhaustive(uint64_t nq, uint64_t nt) {
+    const int64_t lim = tuning().exhaustive_max_pairs;
+    if (lim <= 0) return false;
+    return (double)nq * (double)nt <= (double)lim;
+}
+
 QueryMode mode_for(sdfx_acceleration_method m, sdfx_sign_method s) {
     switch (m) {
         case SDFX_ACCEL_BVH: return s == SDFX_SIGN_NORMAL ? QM_NORMAL_FOLD : QM_RAYCAST_VOTE;
@@ -151,6 +157,8 @@ sdfx_status generate_sdf_core(const float* d_v, uint64_t nv, const uint32_t* d_i
     gather_triangles(d_v, nv, d_idx, topo, nt, tris.get(), flags, s);
     if (method == SDFX_ACCEL_NONE) {
         none_query(tris.get(), nt, d_q, nq, sign, d_out, flags, A);
+    } else if (nt > 0 && use_exhaustive(nq, nt)) {
+        exhaustive_query(tris.get(), nt, d_q, nq, mode_for(method, sign), d_out, flags, A);
     } else if (nt > 0) {
         lbvh_build(tris.get(), nt, bvh, A);
         lbvh_query(bvh, d_q, nq, mode_for(method, sign), d_out, flags, s);

@@ -143,9 +143,10 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
                                                     const SeedBox* __restrict__ boxes,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint64_t* __restrict__ total_ptr, uint64_t* keys,
-                                                    DevFlags* flags) {
+                                                    DevFlags* flags, unsigned long long* evals) {
     const uint64_t total = *total_ptr;
     bool nan_seen = false;
+    uint32_t n_evals = 0;
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
          p += (uint64_t)gridDim.x * blockDim.x) {
         // last triangle whose offset is <= p
@@ -169,6 +170,7 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
         uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
         const TriRec tr = tris[t];
         float d = cell_dist<SIGN>(g, x, y, z, tr);
+        n_evals++;
         nan_seen |= d != d;
         if (key_valid(d)) {
             uint64_t key = make_key<SIGN>(d, (uint32_t)t);
@@ -177,83 +179,93 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
         }
     }
     if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+    if (evals && n_evals) atomicAdd(evals, (unsigned long long)n_evals);
 }
 
 // ---------------------------------------------------------------- propagation
 
-__global__ void __launch_bounds__(256) k_frontier_init(const uint64_t* __restrict__ keys, uint64_t ncells,
-                                                       uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        bool seeded = keys[i] != KEY_EMPTY;
-        uint64_t m = __ballot(seeded);
-        if (!m) continue;
-        uint32_t lane = threadIdx.x & 63u;
-        uint32_t base = 0;
-        int leader = __ffsll((unsigned long long)m) - 1;
-        if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (seeded) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
-    }
-}
-
+// One frontier entry: a cell improved in the previous round, its coordinates
+// and the triangle its key held when the round started.
 struct FrontierItem {
-    uint32_t cell;
     uint32_t tri;
+    uint32_t x, y, z;
 };
 
-__global__ void __launch_bounds__(256) k_frontier_snapshot(const uint32_t* __restrict__ list,
-                                                           const uint32_t* __restrict__ count,
-                                                           uint32_t* __restrict__ next_count,
-                                                           const uint64_t* __restrict__ keys, uint8_t* changed,
-                                                           FrontierItem* __restrict__ items, int sign) {
-    const uint32_t n = *count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t c = list[i];
-        changed[c] = 0;
-        items[i] = FrontierItem{c, key_tri(keys[c], sign)};
+// Marks every seeded cell as changed: round 0 propagates the seeds.
+__global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict__ keys, uint64_t ncells,
+                                                     uint8_t* __restrict__ changed) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        changed[i] = keys[i] != KEY_EMPTY ? 1 : 0;
+}
+
+// Snapshot of the frontier: every changed cell (in cell order within a wave)
+// becomes an item carrying the triangle of its current key; the flag is cleared.
+// Also resets the counter the following compaction appends to.
+__global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __restrict__ keys, GridDev g,
+                                                          uint8_t* __restrict__ changed,
+                                                          FrontierItem* __restrict__ items,
+                                                          uint32_t* __restrict__ count,
+                                                          uint32_t* __restrict__ other_count, int sign) {
+    const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *other_count = 0u;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ncells; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        const bool on = i < ncells && changed[i] != 0;
+        const uint64_t m = __ballot(on);
+        if (!m) continue;
+        uint32_t slot = 0;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        if ((int)lane == leader) slot = atomicAdd(count, (uint32_t)__popcll(m));
+        slot = __shfl(slot, leader, 64);
+        if (on) {
+            changed[i] = 0;
+            const uint32_t z = (uint32_t)(i % g.n[2]);
+            const uint64_t r = i / g.n[2];
+            const uint32_t y = (uint32_t)(r % g.n[1]);
+            const uint32_t x = (uint32_t)(r / g.n[1]);
+            items[slot + __popcll(m & ((1ull << lane) - 1ull))] = FrontierItem{key_tri(keys[i], sign), x, y, z};
+        }
     }
 }
 
+// Tries each item's triangle on the item's 26 neighbours (one lane per pair);
+// a strictly smaller key is stored with atomicMin and flags the cell for the
+// next round. The read before the atomic is a plain (cacheable) load: keys only
+// decrease, so a stale value can only cause an unneeded atomic.
 template <int SIGN>
 __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items,
                                                             const uint32_t* __restrict__ count,
                                                             const TriRec* __restrict__ tris, GridDev g,
-                                                            uint64_t* keys, uint8_t* changed,
-                                                            uint32_t* __restrict__ next_list, uint32_t* next_count,
-                                                            DevFlags* flags) {
+                                                            uint64_t* keys, uint8_t* __restrict__ changed,
+                                                            DevFlags* flags, unsigned long long* evals) {
     const uint64_t work = (uint64_t)(*count) * 26u;
     bool nan_seen = false;
+    uint32_t n_evals = 0;
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < work;
          j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t f = (uint32_t)(j / 26u);
-        uint32_t k = (uint32_t)(j % 26u);
+        const uint32_t jj = (uint32_t)j;  // work < 2^32: at most 2^32 / 26 frontier cells
+        const uint32_t f = jj / 26u;
+        uint32_t k = jj - f * 26u;
         k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
         const int dx = (int)(k / 9u) - 1, dy = (int)((k / 3u) % 3u) - 1, dz = (int)(k % 3u) - 1;
         const FrontierItem it = items[f];
-        const uint32_t cz = it.cell % g.n[2];
-        const uint32_t cy = (it.cell / g.n[2]) % g.n[1];
-        const uint32_t cx = it.cell / (g.n[1] * g.n[2]);
-        const int64_t x = (int64_t)cx + dx, y = (int64_t)cy + dy, z = (int64_t)cz + dz;
+        const int64_t x = (int64_t)it.x + dx, y = (int64_t)it.y + dy, z = (int64_t)it.z + dz;
         if (x < 0 || y < 0 || z < 0 || x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
         const TriRec tr = tris[it.tri];
         float d = cell_dist<SIGN>(g, (uint32_t)x, (uint32_t)y, (uint32_t)z, tr);
+        n_evals++;
         nan_seen |= d != d;
         if (!key_valid(d)) continue;
         const uint64_t key = make_key<SIGN>(d, it.tri);
         const uint64_t ci = cell_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
-        if (key >= __hip_atomic_load(&keys[ci], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        if (key >= keys[ci]) continue;
         const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
-        if (key < old) {
-            // first improver of this round appends the cell to the next frontier
-            uint32_t* word = (uint32_t*)((uintptr_t)&changed[ci] & ~(uintptr_t)3);
-            uint32_t shift = (uint32_t)(((uintptr_t)&changed[ci] & 3u) * 8u);
-            uint32_t prev = atomicOr(word, 1u << shift);
-            if (!((prev >> shift) & 1u)) next_list[atomicAdd(next_count, 1u)] = (uint32_t)ci;
-        }
+        if (key < old) changed[ci] = 1;
     }
     if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+    if (evals && n_evals) atomicAdd(evals, (unsigned long long)n_evals);
 }
 
 // ---------------------------------------------------------------- raycast sign
@@ -400,43 +412,44 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     uint64_t* total = A.alloc<uint64_t>(1);
     SDFX_LAUNCH(k_seed_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, g, boxes, counts);
     exclusive_scan_u64(counts, counts, nt, total, A);
+    // distance-evaluation counters, only while kernel timing is on
+    unsigned long long* evals = nullptr;
+    if (profile_enabled()) {
+        evals = A.alloc<unsigned long long>(2);
+        fill_u32(reinterpret_cast<uint32_t*>(evals), 4, 0u, s);
+    }
     SDFX_LAUNCH(k_seed_pairs<SIGN>, dim3(8192), dim3(256), 0, s, d_tris, nt, g, boxes, counts, total, keys,
-                       d_flags);
+                       d_flags, evals);
     SDFX_HIP(hipGetLastError());
 
     // 2. frontier rounds
-    uint32_t* lists[2] = {A.alloc<uint32_t>(ncells), A.alloc<uint32_t>(ncells)};
+    if (ncells >= (1ull << 32) / 26) throw Error(SDFX_ERR_INVALID_ARGUMENT, "grid too large for one frontier");
     uint32_t* cnt = A.alloc<uint32_t>(2);
-    uint8_t* changed = A.alloc<uint8_t>((ncells + 3) & ~3ull);
+    uint8_t* changed = A.alloc<uint8_t>(ncells);
     FrontierItem* items = A.alloc<FrontierItem>(ncells);
-    SDFX_LAUNCH(k_fill_u8, dim3(stride_blocks(ncells)), dim3(256), 0, s, changed, (ncells + 3) & ~3ull,
-                       (uint8_t)0);
-    SDFX_LAUNCH(k_fill_u32g, dim3(1), dim3(64), 0, s, cnt, 2ull, 0u);
-    SDFX_LAUNCH(k_frontier_init, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, lists[0], cnt);
+    fill_u32(cnt, 2, 0u, s);
+    SDFX_LAUNCH(k_mark_seeded, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, changed);
     SDFX_HIP(hipGetLastError());
     uint32_t* h_cnt = pinned_u32();
     const unsigned prop_blocks = 4096;
-    int cur = 0;
     const int batch = 8;
     // Every round strictly lowers some key, so the loop ends; the cap only turns
     // an inconsistency into an error instead of a hang.
     const uint64_t max_rounds = 64ull * (g.n[0] + g.n[1] + g.n[2]) + 1024;
-    {
-        for (uint64_t round = 0;;) {
-            if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
-            for (int b = 0; b < batch; ++b, ++round) {
-                int nxt = cur ^ 1;
-                SDFX_LAUNCH(k_frontier_snapshot, dim3(2048), dim3(256), 0, s, lists[cur], cnt + cur,
-                                   cnt + nxt, keys, changed, items, (int)SIGN);
-                SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items,
-                                   cnt + cur, d_tris, g, keys, changed, lists[nxt], cnt + nxt, d_flags);
-                cur = nxt;
-            }
-            SDFX_HIP(hipGetLastError());
-            SDFX_HIP(hipMemcpyAsync(h_cnt, cnt + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            SDFX_HIP(hipStreamSynchronize(s));
-            if (*h_cnt == 0) break;
+    int cur = 0;
+    for (uint64_t round = 0;;) {
+        if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
+        for (int b = 0; b < batch; ++b, ++round) {
+            SDFX_LAUNCH(k_frontier_compact, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, changed, items,
+                        cnt + cur, cnt + (cur ^ 1), (int)SIGN);
+            SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, cnt + cur, d_tris, g,
+                        keys, changed, d_flags, evals ? evals + 1 : nullptr);
+            cur ^= 1;
         }
+        SDFX_HIP(hipGetLastError());
+        SDFX_HIP(hipMemcpyAsync(h_cnt, cnt + (cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SDFX_HIP(hipStreamSynchronize(s));
+        if (*h_cnt == 0) break;
     }
 
     // 3. sign
@@ -461,6 +474,13 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     SDFX_LAUNCH(k_grid_finish<SIGN>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, bx, by, bz,
                        d_out);
     SDFX_HIP(hipGetLastError());
+    if (evals) {
+        unsigned long long h[2] = {0, 0};
+        SDFX_HIP(hipMemcpyAsync(h, evals, sizeof h, hipMemcpyDeviceToHost, s));
+        SDFX_HIP(hipStreamSynchronize(s));
+        profile_count("grid.seed_distance_evals", h[0]);
+        profile_count("grid.propagate_distance_evals", h[1]);
+    }
 }
 
 void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,

@@ -71,6 +71,20 @@ void lbvh_query(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, Que
 void lbvh_closest_vectors(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, float* d_vectors,
                           uint8_t* d_inside, DevFlags* d_flags, hipStream_t stream);
 
+// Exhaustive forms of the LBVH query modes (bit-identical results), used when
+// queries x triangles is small enough that no acceleration structure pays.
+void exhaustive_query(const TriRec* d_tris, uint64_t n_tris, const float* d_queries, uint64_t n_queries,
+                      QueryMode mode, float* d_out, DevFlags* d_flags, Arena& arena);
+
+// ---------------------------------------------------------------- tuning
+
+// Process-wide knobs (sdfx_set_tuning). exhaustive_max_pairs: queries x
+// triangles at or below which the accelerated modes run exhaustively.
+struct Tuning {
+    int64_t exhaustive_max_pairs;
+};
+Tuning& tuning();
+
 // ---------------------------------------------------------------- grid
 
 struct GridParams {

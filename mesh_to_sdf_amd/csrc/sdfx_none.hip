@@ -93,6 +93,133 @@ __global__ void __launch_bounds__(NONE_THREADS) k_none_normal(const TriRec* __re
     out[i] = acc;
 }
 
+// ------------------------------------------------------------------ exhaustive
+// forms of the accelerated modes. By the conservativeness of the reference's
+// BVH/R-tree searches their results are functions of all triangles: the
+// minimum d^2 (lowest topology index on ties) and the +X/+Y/+Z crossing
+// parities. For small query x triangle counts these kernels replace the LBVH
+// (no build, every CU busy through triangle-range splitting).
+
+// Per query: min d^2 (as bits, with the topology index in the low word for the
+// nearest-triangle modes) and a 3-bit crossing-parity mask.
+// KIND: 1 = keep the nearest triangle's index, 2 = accumulate crossing parities
+template <int KIND>
+__global__ void __launch_bounds__(NONE_THREADS) k_exhaustive(const TriRec* __restrict__ tris, uint64_t nt,
+                                                              uint64_t chunk, const float* __restrict__ q,
+                                                              uint64_t nq, unsigned long long* __restrict__ best,
+                                                              uint32_t* __restrict__ parity) {
+    uint64_t i = (uint64_t)blockIdx.x * NONE_THREADS + threadIdx.x;
+    bool active = i < nq;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (active) {
+        px = q[3 * i];
+        py = q[3 * i + 1];
+        pz = q[3 * i + 2];
+    }
+    uint64_t t0 = (uint64_t)blockIdx.y * chunk;
+    uint64_t t1 = t0 + chunk < nt ? t0 + chunk : nt;
+    float bd = INFINITY;
+    uint32_t bi = 0xFFFFFFFFu;
+    uint32_t par = 0;
+    constexpr bool WANT_INDEX = (KIND & 1) != 0;
+    constexpr bool WANT_PARITY = (KIND & 2) != 0;
+    for (uint64_t t = t0; t < t1; ++t) {
+        const TriRec tr = tris[t];
+        float dx, dy, dz, cx, cy, cz;
+        closest_diff(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
+        float d2 = dot3(dx, dy, dz, dx, dy, dz);
+        if (d2 < bd) {  // ascending t: the first (lowest index) of equal minima stays
+            bd = d2;
+            bi = (uint32_t)t;
+        }
+        if (WANT_PARITY) {
+            par ^= ray_hit<0>(px, py, pz, tr, nullptr) ? 1u : 0u;
+            par ^= ray_hit<1>(px, py, pz, tr, nullptr) ? 2u : 0u;
+            par ^= ray_hit<2>(px, py, pz, tr, nullptr) ? 4u : 0u;
+        }
+    }
+    if (!active) return;
+    if (bd < INFINITY) {
+        unsigned long long key = ((unsigned long long)fbits(bd) << 32) | (WANT_INDEX ? bi : 0u);
+        if (key < best[i]) atomicMin(&best[i], key);
+    }
+    if (WANT_PARITY && par) atomicXor(&parity[i], par);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(NONE_THREADS) k_exhaustive_finish(const TriRec* __restrict__ tris,
+                                                                     const float* __restrict__ q, uint64_t nq,
+                                                                     const unsigned long long* __restrict__ best,
+                                                                     const uint32_t* __restrict__ parity,
+                                                                     float* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * NONE_THREADS + threadIdx.x;
+    if (i >= nq) return;
+    unsigned long long b = best[i];
+    float d2 = bitsf((uint32_t)(b >> 32));
+    if (MODE == QM_NEAREST_SIGNED) {
+        uint32_t t = (uint32_t)b;
+        out[i] = b != ~0ull ? tri_signed_dist(q[3 * i], q[3 * i + 1], q[3 * i + 2], tris[t]) : sqrtf(d2);
+        return;
+    }
+    float d = sqrtf(b == ~0ull ? INFINITY : d2);
+    if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);
+    uint32_t p = parity[i];
+    int insides = (int)(p & 1u) + (int)((p >> 1) & 1u) + (int)((p >> 2) & 1u);
+    out[i] = insides > 1 ? -d : d;
+}
+
+__global__ void k_fill_u64n(unsigned long long* p, uint64_t n, unsigned long long v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void exhaustive_query(const TriRec* d_tris, uint64_t nt, const float* d_q, uint64_t nq, QueryMode mode,
+                      float* d_out, DevFlags* d_flags, Arena& A) {
+    if (!nq) return;
+    hipStream_t s = A.stream;
+    if (mode == QM_NORMAL_FOLD) {  // the fold is order-dependent: one lane folds all triangles in order
+        none_query(d_tris, nt, d_q, nq, SDFX_SIGN_NORMAL, d_out, d_flags, A);
+        return;
+    }
+    unsigned qblocks = blocks_for(nq, NONE_THREADS);
+    const uint64_t target_blocks = 2048, min_chunk = 32;
+    uint64_t chunks = 1;
+    if (qblocks < target_blocks && nt > min_chunk) {
+        chunks = (target_blocks + qblocks - 1) / qblocks;
+        uint64_t max_chunks = (nt + min_chunk - 1) / min_chunk;
+        if (chunks > max_chunks) chunks = max_chunks;
+        if (chunks > 65535) chunks = 65535;
+    }
+    uint64_t chunk = nt ? (nt + chunks - 1) / chunks : 1;
+    chunks = nt ? (nt + chunk - 1) / chunk : 1;
+    unsigned long long* best = A.alloc<unsigned long long>(nq);
+    uint32_t* parity = A.alloc<uint32_t>(nq);
+    SDFX_LAUNCH(k_fill_u64n, dim3(blocks_for(nq, 256)), dim3(256), 0, s, best, nq, ~0ull);
+    SDFX_LAUNCH(k_fill_u32, dim3(blocks_for(nq, 256)), dim3(256), 0, s, parity, nq, 0u);
+    dim3 grid(qblocks, (unsigned)chunks);
+    if (mode == QM_NEAREST_SIGNED)
+        SDFX_LAUNCH(k_exhaustive<1>, grid, dim3(NONE_THREADS), 0, s, d_tris, nt, chunk, d_q, nq, best,
+                    parity);
+    else
+        SDFX_LAUNCH(k_exhaustive<2>, grid, dim3(NONE_THREADS), 0, s, d_tris, nt, chunk, d_q, nq, best,
+                    parity);
+    switch (mode) {
+        case QM_NEAREST_SIGNED:
+            SDFX_LAUNCH(k_exhaustive_finish<QM_NEAREST_SIGNED>, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, d_q,
+                        nq, best, parity, d_out);
+            break;
+        case QM_RAYCAST_VOTE:
+            SDFX_LAUNCH(k_exhaustive_finish<QM_RAYCAST_VOTE>, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, d_q,
+                        nq, best, parity, d_out);
+            break;
+        default:
+            SDFX_LAUNCH(k_exhaustive_finish<QM_NEAREST_VOTE>, dim3(qblocks), dim3(NONE_THREADS), 0, s, d_tris, d_q,
+                        nq, best, parity, d_out);
+            break;
+    }
+    SDFX_HIP(hipGetLastError());
+}
+
 void none_query(const TriRec* d_tris, uint64_t nt, const float* d_q, uint64_t nq, sdfx_sign_method sign,
                 float* d_out, DevFlags* d_flags, Arena& A) {
     if (!nq) return;

@@ -112,6 +112,19 @@ void dev_free(void* p) {
     P.live.erase(it);
 }
 
+// ---------------------------------------------------------------- tuning
+
+}  // namespace sdfx
+#include "sdfx_internal.hpp"
+namespace sdfx {
+
+Tuning& tuning() {
+    // 2^26 pairs: about 10k queries against 6k triangles, where the exhaustive
+    // kernel finishes well before an LBVH build and traversal would.
+    static Tuning t{int64_t(1) << 26};
+    return t;
+}
+
 // ---------------------------------------------------------------- profiling
 
 namespace {
@@ -125,6 +138,7 @@ struct ProfState {
     std::vector<ProfRecord> pending;
     std::vector<hipEvent_t> free_events;
     std::map<std::string, std::pair<double, uint64_t>> totals;  // name -> (ms, launches)
+    std::map<std::string, uint64_t> counters;
 };
 ProfState& prof() {
     static ProfState* p = new ProfState();
@@ -157,6 +171,13 @@ void resolve(ProfState& P) {
 
 bool profile_enabled() { return prof().enabled; }
 
+void profile_count(const char* name, uint64_t value) {
+    ProfState& P = prof();
+    if (!P.enabled) return;
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.counters[name] += value;
+}
+
 ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
     ProfState& P = prof();
     if (!P.enabled) return;
@@ -184,7 +205,29 @@ void sdfx_profile_enable(int on) {
     std::lock_guard<std::mutex> lk(P.mu);
     sdfx::resolve(P);
     P.totals.clear();
+    P.counters.clear();
     P.enabled = on != 0;
+}
+
+int sdfx_set_tuning(const char* key, int64_t value) {
+    if (!key) return -1;
+    if (std::string(key) == "exhaustive_max_pairs") {
+        sdfx::tuning().exhaustive_max_pairs = value;
+        return 0;
+    }
+    return -1;
+}
+
+int64_t sdfx_get_tuning(const char* key) {
+    if (key && std::string(key) == "exhaustive_max_pairs") return sdfx::tuning().exhaustive_max_pairs;
+    return -1;
+}
+
+uint64_t sdfx_profile_counter(const char* name) {
+    auto& P = sdfx::prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.counters.find(name ? name : "");
+    return it == P.counters.end() ? 0 : it->second;
 }
 
 int sdfx_profile_read(int max_entries, char* names, double* total_ms, uint64_t* launches) {

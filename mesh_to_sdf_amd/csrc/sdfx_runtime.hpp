@@ -115,6 +115,8 @@ int current_device();
 // ProfScope records a HIP event pair around the launches it encloses on the
 // given stream; profile_read() resolves them into per-name totals.
 bool profile_enabled();
+// Adds to a named work counter (e.g. distance evaluations) while profiling.
+void profile_count(const char* name, uint64_t value);
 struct ProfScope {
     ProfScope(const char* name, hipStream_t stream);
     ~ProfScope();

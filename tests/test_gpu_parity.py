@@ -26,6 +26,13 @@ def assert_bit_exact(gpu, ref, what):
 
 
 METHODS = [(0, 0), (0, 1), (1, 0), (1, 1), (2, 1), (3, 0)]
+PATHS = {"lbvh": 0, "exhaustive": 1 << 62}  # exhaustive_max_pairs forcing each implementation
+
+
+@pytest.fixture(params=sorted(PATHS))
+def path(request, sdf):
+    with sdf._lib.tuned("exhaustive_max_pairs", PATHS[request.param]):
+        yield request.param
 
 
 def _accel(sdf, accel, sign):
@@ -42,7 +49,7 @@ def _meshes():
 
 @pytest.mark.parametrize("mesh", ["icosphere2", "torus32x16", "noise_sphere3"])
 @pytest.mark.parametrize("accel,sign", METHODS)
-def test_point_methods_bit_exact(sdf, oracle, mesh, accel, sign):
+def test_point_methods_bit_exact(sdf, oracle, mesh, accel, sign, path):
     v, f = _meshes()[mesh]
     diag = S.bbox_diagonal(v)
     q = np.concatenate([S.uniform_queries(v, 3000, 101), S.surface_queries(v, f, 3000, 0.01 * diag, 102)])
@@ -52,7 +59,7 @@ def test_point_methods_bit_exact(sdf, oracle, mesh, accel, sign):
     assert_bit_exact(gpu, ref, f"{mesh} accel={accel} sign={sign}")
 
 
-def test_suzanne_all_methods_bit_exact(sdf, oracle, suzanne):
+def test_suzanne_all_methods_bit_exact(sdf, oracle, suzanne, path):
     v, idx = suzanne
     f = idx.reshape(-1, 3)
     q = np.concatenate([S.uniform_queries(v, 2000, 103), S.surface_queries(v, f, 2000, 0.02, 104)])
@@ -63,7 +70,7 @@ def test_suzanne_all_methods_bit_exact(sdf, oracle, suzanne):
 
 
 @pytest.mark.parametrize("accel,sign", METHODS)
-def test_topologies(sdf, oracle, accel, sign):
+def test_topologies(sdf, oracle, accel, sign, path):
     """Strips and index-free lists decode like lib.rs:193-215."""
     g = np.random.Generator(np.random.PCG64(5))
     v = g.uniform(-1, 1, size=(40, 3)).astype(np.float32)
@@ -77,7 +84,7 @@ def test_topologies(sdf, oracle, accel, sign):
         assert_bit_exact(gpu, ref, f"topo={topo} idx={'yes' if idx is not None else 'no'}")
 
 
-def test_config1_rtree_bvh_full(sdf, oracle):
+def test_config1_rtree_bvh_full(sdf, oracle, path):
     """Unit icosphere L3, 10,000 uniform queries, RtreeBvh: the full workload."""
     v, f, q = S.config1()
     s, ref = oracle.generate_sdf(v, f, oracle.TOPO_LIST, q, oracle.ACCEL_RTREE_BVH, oracle.SIGN_RAYCAST)
@@ -98,7 +105,7 @@ def test_config3_shape_reduced(sdf, oracle):
     assert err.max() < 1e-6
 
 
-def test_resident_mesh_matches_generate(sdf, oracle):
+def test_resident_mesh_matches_generate(sdf, oracle, path):
     """query_sdf(build(..)) is identical to generate_sdf (e.g. generic/rtree_bvh.rs:331-334)."""
     v, f = S.noise_sphere(3, 0.05)
     q = S.uniform_queries(v, 5000, 105)
