@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="grid", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the HIP-event kernel-timing pass (for runs under an external profiler)")
     return ap.parse_args()
 
 
@@ -116,10 +118,15 @@ def main():
 
     # Kernel timing: the same K steps again with HIP events around every launch
     # (on the launch stream), plus exact work counters.
+    if args.no_kernel_timing:
+        print(json.dumps({"value": value, "ms_per_step": ms, "steps": args.steps}), flush=True)
+        return
     _lib.profile_enable(True)
     timed(args.steps)
     prof = _lib.profile_read()
-    counters = {k: _lib.profile_counter(k) for k in ("grid.seed_distance_evals", "grid.propagate_distance_evals")}
+    counters = {k: _lib.profile_counter(k) for k in (
+        "grid.seed_distance_evals", "grid.propagate_distance_evals", "lbvh.nearest_nodes", "lbvh.nearest_leaves",
+        "lbvh.vote_nodes", "lbvh.vote_leaves", "lbvh.queries")}
     _lib.profile_enable(False)
     kernels = sorted(((name, t, n) for name, (t, n) in prof.items()), key=lambda x: -x[1])
     dom_name, dom_ms, dom_launches = kernels[0]
@@ -177,6 +184,10 @@ def main():
     }
     if roofline_hbm_step:
         result["roofline_hbm_step"] = roofline_hbm_step
+    if counters["lbvh.queries"]:
+        nq_total = counters["lbvh.queries"]
+        result["traversal_per_query"] = {k.split(".", 1)[1]: counters[k] / nq_total for k in (
+            "lbvh.nearest_nodes", "lbvh.nearest_leaves", "lbvh.vote_nodes", "lbvh.vote_leaves")}
     if cpu and cpu.get("speedup") is None:
         cpu["speedup"] = value / cpu["value"] if cpu["value"] else None
     print(json.dumps(result), flush=True)

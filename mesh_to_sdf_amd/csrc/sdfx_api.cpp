@@ -161,7 +161,7 @@ sdfx_status generate_sdf_core(const float* d_v, uint64_t nv, const uint32_t* d_i
         exhaustive_query(tris.get(), nt, d_q, nq, mode_for(method, sign), d_out, flags, A);
     } else if (nt > 0) {
         lbvh_build(tris.get(), nt, bvh, A);
-        lbvh_query(bvh, d_q, nq, mode_for(method, sign), d_out, flags, s);
+        lbvh_query(bvh, d_q, nq, mode_for(method, sign), d_out, flags, A);
     }
     DevFlags f = read_flags(flags, s);
     if (f.non_finite_query) fail_non_finite();
@@ -391,7 +391,7 @@ sdfx_status sdfx_query_sdf(const sdfx_mesh* mesh, const float* query_points, uin
                 none_query(mesh->tris.get(), mesh->n_tris, dq.get(), query_count, mesh->sign, dout.get(), flags, A);
             else
                 lbvh_query(mesh->bvh, dq.get(), query_count, mode_for(mesh->method, mesh->sign), dout.get(), flags,
-                           s);
+                           A);
             DevFlags f = read_flags(flags, s);
             check_internal(f);
             if (f.nan_distance) fail_nan();
@@ -420,7 +420,7 @@ sdfx_status sdfx_query_vector_to_closest_point(const sdfx_mesh* mesh, const floa
         {
             Arena A(s);
             DevFlags* flags = new_flags(A);
-            lbvh_closest_vectors(mesh->bvh, dq.get(), query_count, dvec.get(), din.get(), flags, s);
+            lbvh_closest_vectors(mesh->bvh, dq.get(), query_count, dvec.get(), din.get(), flags, A);
             check_internal(read_flags(flags, s));
         }
         SDFX_HIP(hipMemcpyAsync(vectors_out, dvec.get(), 3 * query_count * sizeof(float), hipMemcpyDeviceToHost, s));

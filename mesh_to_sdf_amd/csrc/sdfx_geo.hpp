@@ -143,10 +143,70 @@ SDFX_HD void closest_diff(float px, float py, float pz, const TriRec& t, float& 
     dx = px - cpx; dy = py - cpy; dz = pz - cpz;
 }
 
+// The same closest point without per-lane branches on the Voronoi region:
+// d1..d6 and va, vb, vc are always formed (the reference forms them lazily;
+// their values do not depend on it), the region is the first of the
+// reference's tests that holds (A, B, C, AB, AC, BC, else interior), and one
+// IEEE division runs on the region's numerator and denominator:
+//   AB: d1 / (d1 - d3)   AC: d2 / (d2 - d6)   BC: (d4 - d3) / ((d4 - d3) + (d5 - d6))
+//   interior: 1 / (va + vb + vc)
+// so every value, including the closest point, is bit-identical to
+// closest_diff. Lanes of a wave no longer serialize on different regions.
+SDFX_HD void closest_diff_flat(float px, float py, float pz, const TriRec& t, float& dx, float& dy, float& dz,
+                               float& cpx, float& cpy, float& cpz) {
+    if (t.degen != 0u) {
+        closest_diff(px, py, pz, t, dx, dy, dz, cpx, cpy, cpz);
+        return;
+    }
+    const float apx = px - t.ax, apy = py - t.ay, apz = pz - t.az;
+    const float bpx = px - t.bx, bpy = py - t.by, bpz = pz - t.bz;
+    const float cqx = px - t.cx, cqy = py - t.cy, cqz = pz - t.cz;
+    const float d1 = dot3(t.abx, t.aby, t.abz, apx, apy, apz);
+    const float d2 = dot3(t.acx, t.acy, t.acz, apx, apy, apz);
+    const float d3 = dot3(t.abx, t.aby, t.abz, bpx, bpy, bpz);
+    const float d4 = dot3(t.acx, t.acy, t.acz, bpx, bpy, bpz);
+    const float d5 = dot3(t.abx, t.aby, t.abz, cqx, cqy, cqz);
+    const float d6 = dot3(t.acx, t.acy, t.acz, cqx, cqy, cqz);
+    const float vc = d1 * d4 - d3 * d2;
+    const float vb = d5 * d2 - d1 * d6;
+    const float va = d3 * d6 - d5 * d4;
+    const float e43 = d4 - d3, e56 = d5 - d6;
+    const bool rA = d1 <= 0.0f && d2 <= 0.0f;
+    const bool rB = !rA && d3 >= 0.0f && d4 <= d3;
+    const bool rC = !rA && !rB && d6 >= 0.0f && d5 <= d6;
+    const bool vert = rA || rB || rC;
+    const bool rAB = !vert && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f;
+    const bool rAC = !vert && !rAB && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
+    const bool rBC = !vert && !rAB && !rAC && va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f;
+    const bool inner = !vert && !rAB && !rAC && !rBC;
+    const float num = rAB ? d1 : (rAC ? d2 : (rBC ? e43 : 1.0f));
+    const float den = rAB ? (d1 - d3) : (rAC ? (d2 - d6) : (rBC ? (e43 + e56) : ((va + vb) + vc)));
+    const float qq = num / den;
+    const float s1 = inner ? vb * qq : qq;
+    const float bx = rBC ? t.bx : t.ax, by = rBC ? t.by : t.ay, bz = rBC ? t.bz : t.az;
+    const float ex = rAC ? t.acx : (rBC ? t.bcx : t.abx);
+    const float ey = rAC ? t.acy : (rBC ? t.bcy : t.aby);
+    const float ez = rAC ? t.acz : (rBC ? t.bcz : t.abz);
+    float qx = bx + ex * s1, qy = by + ey * s1, qz = bz + ez * s1;
+    if (inner) {
+        const float w = vc * qq;
+        qx = qx + t.acx * w;
+        qy = qy + t.acy * w;
+        qz = qz + t.acz * w;
+    }
+    cpx = rA ? t.ax : (rB ? t.bx : (rC ? t.cx : qx));
+    cpy = rA ? t.ay : (rB ? t.by : (rC ? t.cy : qy));
+    cpz = rA ? t.az : (rB ? t.bz : (rC ? t.cz : qz));
+    // vertex regions: p - a etc. are the very values p.sub(vertex) gives
+    dx = rA ? apx : (rB ? bpx : (rC ? cqx : px - qx));
+    dy = rA ? apy : (rB ? bpy : (rC ? cqy : py - qy));
+    dz = rA ? apz : (rB ? bpz : (rC ? cqz : pz - qz));
+}
+
 // geo.rs:33-37 point_triangle_distance2
 SDFX_HD float tri_dist2(float px, float py, float pz, const TriRec& t) {
     float dx, dy, dz, cx, cy, cz;
-    closest_diff(px, py, pz, t, dx, dy, dz, cx, cy, cz);
+    closest_diff_flat(px, py, pz, t, dx, dy, dz, cx, cy, cz);
     return dot3(dx, dy, dz, dx, dy, dz);
 }
 
@@ -157,7 +217,7 @@ SDFX_HD float tri_dist(float px, float py, float pz, const TriRec& t) { return s
 // (so a zero dot product gives -d, and d == 0 gives -0.0).
 SDFX_HD float tri_signed_dist(float px, float py, float pz, const TriRec& t) {
     float dx, dy, dz, cx, cy, cz;
-    closest_diff(px, py, pz, t, dx, dy, dz, cx, cy, cz);
+    closest_diff_flat(px, py, pz, t, dx, dy, dz, cx, cy, cz);
     float d = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
     return dot3(dx, dy, dz, t.nx, t.ny, t.nz) > 0.0f ? d : -d;
 }

@@ -24,6 +24,8 @@
 //    row gives the parity of hits at or beyond each cell, i.e. of the
 //    reference's `+1 on cells 0..=k` counters. The 2-of-3 vote is fused into
 //    the output write.
+#include <algorithm>
+
 #include "sdfx_internal.hpp"
 
 namespace sdfx {
@@ -179,10 +181,13 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
         }
     }
     if (nan_seen) atomicOr(&flags->nan_distance, 1u);
-    if (evals && n_evals) atomicAdd(evals, (unsigned long long)n_evals);
+    if (evals) block_count(evals, n_evals);
 }
 
 // ---------------------------------------------------------------- propagation
+
+constexpr int COMPACT_PER_LANE = 16;
+constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
 
 // One frontier entry: a cell improved in the previous round, its coordinates
 // and the triangle its key held when the round started.
@@ -193,40 +198,100 @@ struct FrontierItem {
 
 // Marks every seeded cell as changed: round 0 propagates the seeds.
 __global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict__ keys, uint64_t ncells,
-                                                     uint8_t* __restrict__ changed) {
+                                                     uint8_t* __restrict__ changed, uint8_t* __restrict__ tile_changed,
+                                                     uint64_t tile) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        changed[i] = keys[i] != KEY_EMPTY ? 1 : 0;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const bool seeded = keys[i] != KEY_EMPTY;
+        changed[i] = seeded ? 1 : 0;
+        if (i % tile == 0) tile_changed[i / tile] = 1;  // conservatively: every tile is scanned once
+    }
 }
 
-// Snapshot of the frontier: every changed cell (in cell order within a wave)
-// becomes an item carrying the triangle of its current key; the flag is cleared.
-// Also resets the counter the following compaction appends to.
+// Snapshot of the frontier: every changed cell becomes an item carrying the
+// triangle of its current key, and its flag is cleared. Each block takes a
+// contiguous tile of COMPACT_TILE cells (16 per lane, read as four u32 words),
+// ranks them with a block-wide prefix sum and reserves its output range with a
+// single atomic, so items stay in cell order within a tile and the shared
+// counter sees one atomic per tile. Also resets the counter the next
+// compaction appends to.
+
 __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __restrict__ keys, GridDev g,
                                                           uint8_t* __restrict__ changed,
+                                                          uint8_t* __restrict__ tile_changed,
                                                           FrontierItem* __restrict__ items,
                                                           uint32_t* __restrict__ count,
                                                           uint32_t* __restrict__ other_count, int sign) {
+    __shared__ uint32_t wave_tot[4];
+    __shared__ uint32_t block_base;
     const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) *other_count = 0u;
-    const uint32_t lane = threadIdx.x & 63u;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ncells; base += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = base + threadIdx.x;
-        const bool on = i < ncells && changed[i] != 0;
-        const uint64_t m = __ballot(on);
-        if (!m) continue;
-        uint32_t slot = 0;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        if ((int)lane == leader) slot = atomicAdd(count, (uint32_t)__popcll(m));
-        slot = __shfl(slot, leader, 64);
-        if (on) {
-            changed[i] = 0;
-            const uint32_t z = (uint32_t)(i % g.n[2]);
-            const uint64_t r = i / g.n[2];
-            const uint32_t y = (uint32_t)(r % g.n[1]);
-            const uint32_t x = (uint32_t)(r / g.n[1]);
-            items[slot + __popcll(m & ((1ull << lane) - 1ull))] = FrontierItem{key_tri(keys[i], sign), x, y, z};
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint64_t tile = (uint64_t)blockIdx.x * COMPACT_TILE; tile < ncells;
+         tile += (uint64_t)gridDim.x * COMPACT_TILE) {
+        // tiles no cell of which improved in the last round hold no frontier cell
+        const uint64_t tile_id = tile / COMPACT_TILE;
+        if (!tile_changed[tile_id]) continue;  // block-uniform
+        __syncthreads();  // every lane has read the flag before it is cleared
+        if (threadIdx.x == 0) tile_changed[tile_id] = 0;
+        const uint64_t c0 = tile + (uint64_t)threadIdx.x * COMPACT_PER_LANE;
+        uint32_t w[COMPACT_PER_LANE / 4];
+        if (c0 + COMPACT_PER_LANE <= ncells) {
+            const uint4 v = *reinterpret_cast<const uint4*>(changed + c0);  // c0 is a multiple of 16
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < COMPACT_PER_LANE / 4; ++k) {
+                uint32_t x = 0;
+                for (int b = 0; b < 4; ++b) {
+                    uint64_t c = c0 + 4 * k + b;
+                    if (c < ncells && changed[c]) x |= 1u << (8 * b);
+                }
+                w[k] = x;
+            }
         }
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER_LANE / 4; ++k) mine += __popc(w[k] & 0x01010101u);
+        // block-wide exclusive prefix of `mine`
+        uint32_t x = mine;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(x, off, 64);
+            if ((int)lane >= off) x += y;
+        }
+        if (lane == 63) wave_tot[wave] = x;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            before += (uint32_t)q < wave ? wave_tot[q] : 0u;
+            total += wave_tot[q];
+        }
+        if (threadIdx.x == 0) block_base = total ? atomicAdd(count, total) : 0u;
+        __syncthreads();
+        uint32_t slot = block_base + before + x - mine;
+        if (mine) {
+#pragma unroll
+            for (int k = 0; k < COMPACT_PER_LANE / 4; ++k) {
+                if (!(w[k] & 0x01010101u)) continue;
+                for (int b = 0; b < 4; ++b) {
+                    if (!((w[k] >> (8 * b)) & 1u)) continue;
+                    const uint64_t i = c0 + 4 * k + b;
+                    const uint32_t z = (uint32_t)(i % g.n[2]);
+                    const uint64_t r = i / g.n[2];
+                    const uint32_t y = (uint32_t)(r % g.n[1]);
+                    const uint32_t xx = (uint32_t)(r / g.n[1]);
+                    items[slot++] = FrontierItem{key_tri(keys[i], sign), xx, y, z};
+                }
+            }
+            if (c0 + COMPACT_PER_LANE <= ncells) {
+                *reinterpret_cast<uint4*>(changed + c0) = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+                for (uint64_t c = c0; c < ncells; ++c) changed[c] = 0;
+            }
+        }
+        __syncthreads();  // wave_tot / block_base are reused by the next tile
     }
 }
 
@@ -239,7 +304,8 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
                                                             const uint32_t* __restrict__ count,
                                                             const TriRec* __restrict__ tris, GridDev g,
                                                             uint64_t* keys, uint8_t* __restrict__ changed,
-                                                            DevFlags* flags, unsigned long long* evals) {
+                                                            uint8_t* __restrict__ tile_changed, DevFlags* flags,
+                                                            unsigned long long* evals) {
     const uint64_t work = (uint64_t)(*count) * 26u;
     bool nan_seen = false;
     uint32_t n_evals = 0;
@@ -253,19 +319,26 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
         const FrontierItem it = items[f];
         const int64_t x = (int64_t)it.x + dx, y = (int64_t)it.y + dy, z = (int64_t)it.z + dz;
         if (x < 0 || y < 0 || z < 0 || x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
+        const uint64_t ci = cell_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+        const uint64_t cur = keys[ci];
+        // A neighbour already holding this triangle holds exactly the key the
+        // triangle would give it, and keys only decrease: nothing to try.
+        if (cur != KEY_EMPTY && key_tri(cur, SIGN) == it.tri) continue;
         const TriRec tr = tris[it.tri];
         float d = cell_dist<SIGN>(g, (uint32_t)x, (uint32_t)y, (uint32_t)z, tr);
         n_evals++;
         nan_seen |= d != d;
         if (!key_valid(d)) continue;
         const uint64_t key = make_key<SIGN>(d, it.tri);
-        const uint64_t ci = cell_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
-        if (key >= keys[ci]) continue;
+        if (key >= cur) continue;
         const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
-        if (key < old) changed[ci] = 1;
+        if (key < old) {
+            changed[ci] = 1;
+            tile_changed[ci / COMPACT_TILE] = 1;
+        }
     }
     if (nan_seen) atomicOr(&flags->nan_distance, 1u);
-    if (evals && n_evals) atomicAdd(evals, (unsigned long long)n_evals);
+    if (evals) block_count(evals, n_evals);
 }
 
 // ---------------------------------------------------------------- raycast sign
@@ -379,18 +452,37 @@ __global__ void __launch_bounds__(256) k_grid_finish(const uint64_t* __restrict_
 
 // ---------------------------------------------------------------- driver
 
-// One pinned word per thread for reading the frontier size back.
-static uint32_t* pinned_u32() {
-    struct Holder {
-        uint32_t* p = nullptr;
-        ~Holder() {
-            if (p) (void)hipHostFree(p);
+// Two pinned words per thread and their events: the host reads the frontier
+// size of one batch while the next batch runs.
+struct BatchProbe {
+    struct Slots {
+        uint32_t* words = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Slots() {
+            if (words) (void)hipHostFree(words);
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
         }
     };
-    static thread_local Holder h;
-    if (!h.p) SDFX_HIP(hipHostMalloc((void**)&h.p, sizeof(uint32_t), hipHostMallocDefault));
-    return h.p;
-}
+    Slots& slots() {
+        static thread_local Slots sl;
+        if (!sl.words) {
+            SDFX_HIP(hipHostMalloc((void**)&sl.words, 2 * sizeof(uint32_t), hipHostMallocDefault));
+            for (auto& e : sl.ev) SDFX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        return sl;
+    }
+    void record(int slot, const uint32_t* d_count, hipStream_t s) {
+        Slots& sl = slots();
+        SDFX_HIP(hipMemcpyAsync(&sl.words[slot], d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SDFX_HIP(hipEventRecord(sl.ev[slot], s));
+    }
+    uint32_t wait(int slot) {
+        Slots& sl = slots();
+        SDFX_HIP(hipEventSynchronize(sl.ev[slot]));
+        return sl.words[slot];
+    }
+};
 
 static unsigned stride_blocks(uint64_t n) {
     uint64_t b = (n + 255) / 256;
@@ -413,10 +505,12 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     SDFX_LAUNCH(k_seed_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, g, boxes, counts);
     exclusive_scan_u64(counts, counts, nt, total, A);
     // distance-evaluation counters, only while kernel timing is on
+    // per-block slots: seeding (8192 blocks), then propagation (4096 blocks)
     unsigned long long* evals = nullptr;
+    const uint64_t seed_slots = 8192, prop_slots = 4096;
     if (profile_enabled()) {
-        evals = A.alloc<unsigned long long>(2);
-        fill_u32(reinterpret_cast<uint32_t*>(evals), 4, 0u, s);
+        evals = A.alloc<unsigned long long>(seed_slots + prop_slots);
+        fill_u32(reinterpret_cast<uint32_t*>(evals), 2 * (seed_slots + prop_slots), 0u, s);
     }
     SDFX_LAUNCH(k_seed_pairs<SIGN>, dim3(8192), dim3(256), 0, s, d_tris, nt, g, boxes, counts, total, keys,
                        d_flags, evals);
@@ -424,32 +518,42 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
 
     // 2. frontier rounds
     if (ncells >= (1ull << 32) / 26) throw Error(SDFX_ERR_INVALID_ARGUMENT, "grid too large for one frontier");
+    const uint64_t ntiles = (ncells + COMPACT_TILE - 1) / COMPACT_TILE;
     uint32_t* cnt = A.alloc<uint32_t>(2);
-    uint8_t* changed = A.alloc<uint8_t>(ncells);
+    uint8_t* changed = A.alloc<uint8_t>((ncells + 15) & ~15ull);  // 16-byte vector reads
+    uint8_t* tile_changed = A.alloc<uint8_t>(ntiles);
     FrontierItem* items = A.alloc<FrontierItem>(ncells);
     fill_u32(cnt, 2, 0u, s);
-    SDFX_LAUNCH(k_mark_seeded, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, changed);
+    SDFX_LAUNCH(k_mark_seeded, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, changed, tile_changed,
+                (uint64_t)COMPACT_TILE);
     SDFX_HIP(hipGetLastError());
-    uint32_t* h_cnt = pinned_u32();
     const unsigned prop_blocks = 4096;
+    const unsigned compact_blocks = (unsigned)std::min<uint64_t>(ntiles, 4096);
     const int batch = 8;
     // Every round strictly lowers some key, so the loop ends; the cap only turns
     // an inconsistency into an error instead of a hang.
     const uint64_t max_rounds = 64ull * (g.n[0] + g.n[1] + g.n[2]) + 1024;
+    // Batches are enqueued one ahead of the one being checked: while the host
+    // waits for batch k's frontier size, batch k+1 is already on the GPU.
+    BatchProbe probe;
     int cur = 0;
-    for (uint64_t round = 0;;) {
-        if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
+    uint64_t round = 0;
+    auto enqueue_batch = [&](int slot) {
         for (int b = 0; b < batch; ++b, ++round) {
-            SDFX_LAUNCH(k_frontier_compact, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, changed, items,
-                        cnt + cur, cnt + (cur ^ 1), (int)SIGN);
+            SDFX_LAUNCH(k_frontier_compact, dim3(compact_blocks), dim3(256), 0, s, keys, g, changed, tile_changed,
+                        items, cnt + cur, cnt + (cur ^ 1), (int)SIGN);
             SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, cnt + cur, d_tris, g,
-                        keys, changed, d_flags, evals ? evals + 1 : nullptr);
+                        keys, changed, tile_changed, d_flags, evals ? evals + seed_slots : nullptr);
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());
-        SDFX_HIP(hipMemcpyAsync(h_cnt, cnt + (cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        SDFX_HIP(hipStreamSynchronize(s));
-        if (*h_cnt == 0) break;
+        probe.record(slot, cnt + (cur ^ 1), s);  // size of the batch's last frontier
+    };
+    enqueue_batch(0);
+    for (int slot = 0;; slot ^= 1) {
+        if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
+        enqueue_batch(slot ^ 1);
+        if (probe.wait(slot) == 0) break;  // the batch after it only runs empty rounds
     }
 
     // 3. sign
@@ -475,11 +579,8 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
                        d_out);
     SDFX_HIP(hipGetLastError());
     if (evals) {
-        unsigned long long h[2] = {0, 0};
-        SDFX_HIP(hipMemcpyAsync(h, evals, sizeof h, hipMemcpyDeviceToHost, s));
-        SDFX_HIP(hipStreamSynchronize(s));
-        profile_count("grid.seed_distance_evals", h[0]);
-        profile_count("grid.propagate_distance_evals", h[1]);
+        profile_count("grid.seed_distance_evals", sum_block_counters(evals, seed_slots, s));
+        profile_count("grid.propagate_distance_evals", sum_block_counters(evals + seed_slots, prop_slots, s));
     }
 }
 

@@ -65,11 +65,11 @@ enum QueryMode : int {
 };
 
 void lbvh_query(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, QueryMode mode, float* d_out,
-                DevFlags* d_flags, hipStream_t stream);
+                DevFlags* d_flags, Arena& arena);
 
 // generic/rtree_bvh.rs:217-276: closest - query and the 3-ray inside vote.
 void lbvh_closest_vectors(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, float* d_vectors,
-                          uint8_t* d_inside, DevFlags* d_flags, hipStream_t stream);
+                          uint8_t* d_inside, DevFlags* d_flags, Arena& arena);
 
 // Exhaustive forms of the LBVH query modes (bit-identical results), used when
 // queries x triangles is small enough that no acceleration structure pays.
@@ -99,6 +99,22 @@ void grid_sdf(const TriRec* d_tris, uint64_t n_tris, const GridParams& g, sdfx_s
               DevFlags* d_flags, Arena& arena);
 
 // ---------------------------------------------------------------- utilities
+
+// Adds v to out[blockIdx.x] once per block (block-wide sum in LDS). Each block
+// owns its slot, so the counters of consecutive launches on one stream need
+// no atomics; the host sums the slots.
+__device__ __forceinline__ void block_count(unsigned long long* out, uint32_t v) {
+    __shared__ unsigned long long sdfx_block_sum;
+    if (threadIdx.x == 0) sdfx_block_sum = 0;
+    __syncthreads();
+    unsigned long long w = v;
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+    if ((threadIdx.x & 63u) == 0 && w) atomicAdd(&sdfx_block_sum, w);
+    __syncthreads();
+    if (threadIdx.x == 0 && sdfx_block_sum) out[blockIdx.x] += sdfx_block_sum;
+}
+// Sums `n` per-block counter slots on the host (synchronises the stream).
+uint64_t sum_block_counters(const unsigned long long* d_slots, uint64_t n, hipStream_t stream);
 
 // Fills n u32 words with v using a kernel (no memset node, so the call is safe
 // inside captured graphs).

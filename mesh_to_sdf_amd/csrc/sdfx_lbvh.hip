@@ -363,8 +363,14 @@ struct Nearest {
     int32_t leaf;    // Morton position, -1 if none
 };
 
+// Traversal statistics gathered while kernel timing is on.
+struct TravStats {
+    uint32_t nodes = 0, leaves = 0, vote_nodes = 0, vote_leaves = 0;
+};
+
 __device__ __forceinline__ void visit_leaf_nearest(const TriRec* __restrict__ leaves, int32_t leaf, float px,
-                                                   float py, float pz, Nearest& best) {
+                                                   float py, float pz, Nearest& best, TravStats& ts) {
+    ts.leaves++;
     const TriRec t = leaves[leaf];
     float d2 = tri_dist2(px, py, pz, t);
     if (d2 < best.d2 || (d2 == best.d2 && t.index < best.index)) {
@@ -376,10 +382,10 @@ __device__ __forceinline__ void visit_leaf_nearest(const TriRec* __restrict__ le
 
 // Minimum d^2 over all triangles, closest child first.
 __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
-                               float px, float py, float pz, uint32_t max_visits, bool& overflow) {
+                               float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
     Nearest best{INFINITY, 0xFFFFFFFFu, -1};
     if (root < 0) {
-        visit_leaf_nearest(leaves, ~root, px, py, pz, best);
+        visit_leaf_nearest(leaves, ~root, px, py, pz, best, ts);
         return best;
     }
     int32_t stack_node[STACK_DEPTH];
@@ -391,18 +397,19 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
             overflow = true;
             break;
         }
+        ts.nodes++;
         const BvhNode nd = nodes[node];
         float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
         float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
         bool gl = dl * PRUNE_SLACK <= best.d2;
         bool gr = dr * PRUNE_SLACK <= best.d2;
         if (gl && nd.left < 0) {
-            visit_leaf_nearest(leaves, ~nd.left, px, py, pz, best);
+            visit_leaf_nearest(leaves, ~nd.left, px, py, pz, best, ts);
             gl = false;
             gr = dr * PRUNE_SLACK <= best.d2;
         }
         if (gr && nd.right < 0) {
-            visit_leaf_nearest(leaves, ~nd.right, px, py, pz, best);
+            visit_leaf_nearest(leaves, ~nd.right, px, py, pz, best, ts);
             gr = false;
         }
         if (gl && gr) {
@@ -532,7 +539,8 @@ __device__ __forceinline__ uint32_t ray_axes(const float* lo, const float* hi, f
 }
 
 __device__ __forceinline__ void ray_leaf(const TriRec* __restrict__ leaves, int32_t leaf, uint32_t axes, float px,
-                                         float py, float pz, uint32_t* cnt) {
+                                         float py, float pz, uint32_t* cnt, TravStats& ts) {
+    ts.vote_leaves++;
     const TriRec t = leaves[leaf];
     if ((axes & 1u) && ray_hit<0>(px, py, pz, t, nullptr)) cnt[0]++;
     if ((axes & 2u) && ray_hit<1>(px, py, pz, t, nullptr)) cnt[1]++;
@@ -542,10 +550,10 @@ __device__ __forceinline__ void ray_leaf(const TriRec* __restrict__ leaves, int3
 // The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198: the
 // three rays share one traversal; each stack entry carries its live axes.
 __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
-                                float px, float py, float pz, uint32_t max_visits, bool& overflow) {
+                                float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
     uint32_t cnt[3] = {0u, 0u, 0u};
     if (root < 0) {
-        ray_leaf(leaves, ~root, 7u, px, py, pz, cnt);
+        ray_leaf(leaves, ~root, 7u, px, py, pz, cnt, ts);
     } else {
         uint32_t stack[STACK_DEPTH];
         int sp = 0;
@@ -556,15 +564,16 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
                 overflow = true;
                 break;
             }
+            ts.vote_nodes++;
             const BvhNode nd = nodes[node];
             uint32_t al = axes & ray_axes(nd.lmin, nd.lmax, px, py, pz);
             uint32_t ar = axes & ray_axes(nd.rmin, nd.rmax, px, py, pz);
             if (al && nd.left < 0) {
-                ray_leaf(leaves, ~nd.left, al, px, py, pz, cnt);
+                ray_leaf(leaves, ~nd.left, al, px, py, pz, cnt, ts);
                 al = 0;
             }
             if (ar && nd.right < 0) {
-                ray_leaf(leaves, ~nd.right, ar, px, py, pz, cnt);
+                ray_leaf(leaves, ~nd.right, ar, px, py, pz, cnt, ts);
                 ar = 0;
             }
             if (al && ar) {
@@ -598,67 +607,125 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ leaves, int32_t root,
                                                     uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
-                                                    float* __restrict__ out, DevFlags* flags) {
+                                                    const uint32_t* __restrict__ order, float* __restrict__ out,
+                                                    DevFlags* flags, unsigned long long* stats) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
-    bool overflow = false;
-    if (MODE == QM_NORMAL_FOLD) {
-        bool nan_seen = false;
-        float acc = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
-        if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+    const bool active = i < nq;  // no early return: the statistics below use block barriers
+    TravStats ts;
+    if (active) {
+        if (order) i = order[i];  // lanes of a wave take Morton-adjacent queries
+        const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+        bool overflow = false;
+        if (MODE == QM_NORMAL_FOLD) {
+            bool nan_seen = false;
+            out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
+            if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+        } else {
+            Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
+            if (MODE == QM_NEAREST_SIGNED) {
+                out[i] = nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
+            } else {
+                float d = sqrtf(nb.d2);
+                if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
+                out[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? -d : d;
+            }
+        }
         if (overflow) atomicOr(&flags->internal, 1u);
-        out[i] = acc;
-        return;
     }
-    Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow);
-    if (MODE == QM_NEAREST_SIGNED) {
-        out[i] = nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
-    } else {
-        float d = sqrtf(nb.d2);
-        if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
-        out[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow) ? -d : d;
+    if (stats) {  // kernel argument: uniform, every lane of the block gets here
+        const uint64_t nb = gridDim.x;
+        block_count(stats, ts.nodes);
+        block_count(stats + nb, ts.leaves);
+        block_count(stats + 2 * nb, ts.vote_nodes);
+        block_count(stats + 3 * nb, ts.vote_leaves);
     }
-    if (overflow) atomicOr(&flags->internal, 1u);
 }
 
 static uint32_t visit_cap(const Lbvh& bvh) { return (uint32_t)(2 * bvh.n + 64); }
 
+// Morton code of each query inside the tree's root box (clamped outside it).
+__global__ void __launch_bounds__(256) k_query_morton(const BvhNode* __restrict__ nodes, const float* __restrict__ q,
+                                                      uint64_t nq, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const BvhNode r = nodes[0];
+    uint32_t c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lo = fminf(r.lmin[k], r.rmin[k]), hi = fmaxf(r.lmax[k], r.rmax[k]);
+        float ext = hi - lo;
+        float u = ext > 0.0f ? (q[3 * i + k] - lo) / ext : 0.0f;
+        float sc = u * 1024.0f;
+        c[k] = sc <= 0.0f ? 0u : (sc >= 1023.0f ? 1023u : (uint32_t)sc);  // NaN -> 0
+    }
+    keys[i] = (expand10(c[0]) << 2) | (expand10(c[1]) << 1) | expand10(c[2]);
+    vals[i] = (uint32_t)i;
+}
+
+// Query order for coherent traversal: the permutation sorting the queries by
+// Morton code, so a wave's lanes walk nearby paths (null when not worth it).
+static const uint32_t* query_order(const Lbvh& bvh, const float* d_q, uint64_t nq, Arena& A) {
+    if (nq < 4096 || bvh.root < 0 || nq >= (1ull << 32)) return nullptr;
+    uint32_t* keys = A.alloc<uint32_t>(nq);
+    uint32_t* vals = A.alloc<uint32_t>(nq);
+    SDFX_LAUNCH(k_query_morton, dim3(blocks_for(nq, 256)), dim3(256), 0, A.stream, bvh.nodes.get(), d_q, nq, keys,
+                vals);
+    radix_sort_pairs(keys, vals, nq, 30, A);
+    return vals;
+}
+
 void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, float* d_out, DevFlags* d_flags,
-                hipStream_t s) {
+                Arena& A) {
     if (!nq || !bvh.n) return;
+    hipStream_t s = A.stream;
+    const uint32_t* order = query_order(bvh, d_q, nq, A);
     unsigned blocks = blocks_for(nq, 256);
+    unsigned long long* stats = nullptr;
+    if (profile_enabled() && mode != QM_NORMAL_FOLD) {
+        stats = A.alloc<unsigned long long>(4ull * blocks);
+        fill_u32(reinterpret_cast<uint32_t*>(stats), 8ull * blocks, 0u, s);
+    }
     switch (mode) {
         case QM_RAYCAST_VOTE:
             SDFX_LAUNCH(k_lbvh_query<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
             break;
         case QM_NORMAL_FOLD:
             SDFX_LAUNCH(k_lbvh_query<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
             break;
         case QM_NEAREST_SIGNED:
             SDFX_LAUNCH(k_lbvh_query<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
             break;
         case QM_NEAREST_VOTE:
             SDFX_LAUNCH(k_lbvh_query<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_out, d_flags);
+                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
             break;
     }
     SDFX_HIP(hipGetLastError());
+    if (stats) {
+        profile_count("lbvh.nearest_nodes", sum_block_counters(stats, blocks, s));
+        profile_count("lbvh.nearest_leaves", sum_block_counters(stats + blocks, blocks, s));
+        profile_count("lbvh.vote_nodes", sum_block_counters(stats + 2ull * blocks, blocks, s));
+        profile_count("lbvh.vote_leaves", sum_block_counters(stats + 3ull * blocks, blocks, s));
+        profile_count("lbvh.queries", nq);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restrict__ nodes,
                                                          const TriRec* __restrict__ leaves, int32_t root,
                                                          uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
-                                                         float* __restrict__ vec, uint8_t* __restrict__ inside,
-                                                         DevFlags* flags) {
+                                                         const uint32_t* __restrict__ order, float* __restrict__ vec,
+                                                         uint8_t* __restrict__ inside, DevFlags* flags) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
+    if (order) i = order[i];
     float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
     bool overflow = false;
-    Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow);
+    TravStats ts;
+    Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
     float vx = NAN, vy = NAN, vz = NAN;
     if (nb.leaf >= 0) {
         float dx, dy, dz, cx, cy, cz;
@@ -670,15 +737,17 @@ __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restri
     vec[3 * i] = vx;
     vec[3 * i + 1] = vy;
     vec[3 * i + 2] = vz;
-    inside[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow) ? 1 : 0;
+    inside[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? 1 : 0;
     if (overflow) atomicOr(&flags->internal, 1u);
 }
 
 void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float* d_vec, uint8_t* d_inside,
-                          DevFlags* d_flags, hipStream_t s) {
+                          DevFlags* d_flags, Arena& A) {
     if (!nq || !bvh.n) return;
+    hipStream_t s = A.stream;
+    const uint32_t* order = query_order(bvh, d_q, nq, A);
     SDFX_LAUNCH(k_closest_vectors, dim3(blocks_for(nq, 256)), dim3(256), 0, s, bvh.nodes.get(),
-                       bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, d_vec, d_inside, d_flags);
+                       bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_vec, d_inside, d_flags);
     SDFX_HIP(hipGetLastError());
 }
 

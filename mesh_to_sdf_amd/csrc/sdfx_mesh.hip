@@ -94,6 +94,15 @@ void fill_u32(uint32_t* d_ptr, uint64_t n, uint32_t v, hipStream_t s) {
     SDFX_HIP(hipGetLastError());
 }
 
+uint64_t sum_block_counters(const unsigned long long* d_slots, uint64_t n, hipStream_t s) {
+    std::vector<unsigned long long> h(n);
+    SDFX_HIP(hipMemcpyAsync(h.data(), d_slots, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    SDFX_HIP(hipStreamSynchronize(s));
+    uint64_t t = 0;
+    for (auto x : h) t += x;
+    return t;
+}
+
 // ---------------------------------------------------------------- scans
 
 constexpr int SCAN_THREADS = 256;

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(NONE_THREADS) k_none_raycast(const TriRec* __r
     for (uint64_t t = t0; t < t1; ++t) {
         const TriRec tr = tris[t];
         float dx, dy, dz, cx, cy, cz;
-        closest_diff(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
+        closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
         float d2 = dot3(dx, dy, dz, dx, dy, dz);
         if (d2 < best) best = d2;  // NaN never replaces (f32::min ignores NaN)
         hits += ray_hit<0>(px, py, pz, tr, nullptr) ? 1u : 0u;
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(NONE_THREADS) k_exhaustive(const TriRec* __res
     for (uint64_t t = t0; t < t1; ++t) {
         const TriRec tr = tris[t];
         float dx, dy, dz, cx, cy, cz;
-        closest_diff(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
+        closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
         float d2 = dot3(dx, dy, dz, dx, dy, dz);
         if (d2 < bd) {  // ascending t: the first (lowest index) of equal minima stays
             bd = d2;

@@ -151,7 +151,9 @@ hipEvent_t take_event(ProfState& P) {
         return e;
     }
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    // timing-only events: no system-scope fence (cache writeback and
+    // invalidation) when recorded, so the kernels between them run undisturbed
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 void resolve(ProfState& P) {
@@ -178,22 +180,28 @@ void profile_count(const char* name, uint64_t value) {
     P.counters[name] += value;
 }
 
-ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+ProfPair prof_begin(const char* name) {
+    ProfPair p;
     ProfState& P = prof();
-    if (!P.enabled) return;
+    if (!P.enabled) return p;
     std::lock_guard<std::mutex> lk(P.mu);
-    start = take_event(P);
-    if (start) (void)hipEventRecord(start, stream);
+    p.a = take_event(P);
+    p.b = take_event(P);
+    if (!p.a || !p.b) {
+        if (p.a) P.free_events.push_back(p.a);
+        if (p.b) P.free_events.push_back(p.b);
+        p.a = p.b = nullptr;
+        return p;
+    }
+    p.name = name;
+    return p;
 }
 
-ProfScope::~ProfScope() {
-    if (!start) return;
+void prof_end(const ProfPair& p) {
+    if (!p.a) return;
     ProfState& P = prof();
     std::lock_guard<std::mutex> lk(P.mu);
-    hipEvent_t stop = take_event(P);
-    if (!stop) return;
-    (void)hipEventRecord(stop, stream);
-    P.pending.push_back({name, start, stop});
+    P.pending.push_back({p.name, p.a, p.b});
 }
 
 }  // namespace sdfx

@@ -2,6 +2,7 @@
 // status/error plumbing, per-thread streams and a caching device allocator.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -112,26 +113,29 @@ struct DevFlags {
 int current_device();
 
 // Kernel timing for measurement tools (off by default). While enabled, each
-// ProfScope records a HIP event pair around the launches it encloses on the
-// given stream; profile_read() resolves them into per-name totals.
+// launch made through SDFX_LAUNCH goes through hipExtLaunchKernelGGL with a
+// start/stop event pair that the runtime ties to the kernel's own dispatch, so
+// the pair brackets exactly that kernel's execution on its stream;
+// sdfx_profile_read() resolves the pairs into per-name totals.
 bool profile_enabled();
 // Adds to a named work counter (e.g. distance evaluations) while profiling.
 void profile_count(const char* name, uint64_t value);
-struct ProfScope {
-    ProfScope(const char* name, hipStream_t stream);
-    ~ProfScope();
-    ProfScope(const ProfScope&) = delete;
-    ProfScope& operator=(const ProfScope&) = delete;
-    const char* name;
-    hipStream_t stream;
-    hipEvent_t start = nullptr;
+struct ProfPair {
+    const char* name = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
 };
+ProfPair prof_begin(const char* name);  // {nullptr events} when profiling is off
+void prof_end(const ProfPair& p);
 
-// hipLaunchKernelGGL with an optional timing scope named after the kernel.
-#define SDFX_LAUNCH(kernel, grid, block, shmem, stream, ...)                         \
-    do {                                                                             \
-        ::sdfx::ProfScope sdfx_ps_(#kernel, stream);                                 \
-        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);         \
+// hipLaunchKernelGGL, or its timed form while kernel timing is enabled.
+#define SDFX_LAUNCH(kernel, grid, block, shmem, stream, ...)                                                 \
+    do {                                                                                                     \
+        const ::sdfx::ProfPair sdfx_pp_ = ::sdfx::prof_begin(#kernel);                                       \
+        if (sdfx_pp_.a)                                                                                      \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, sdfx_pp_.a, sdfx_pp_.b, 0, __VA_ARGS__); \
+        else                                                                                                 \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                             \
+        ::sdfx::prof_end(sdfx_pp_);                                                                          \
     } while (0)
 
 // Launch helper: blocks for n items with `threads` per block.
