@@ -25,6 +25,7 @@
 //    reference's `+1 on cells 0..=k` counters. The 2-of-3 vote is fused into
 //    the output write.
 #include <algorithm>
+#include <cmath>
 
 #include "sdfx_internal.hpp"
 
@@ -188,144 +189,209 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
 
 constexpr int COMPACT_PER_LANE = 16;
 constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
+// Width of the distance band a round takes, in units of the largest |cell size|.
+constexpr float GRID_BAND_CELLS = 3.0f;
+// Round bookkeeping is spread over NCLASS words, WORD_STRIDE u32 (256 bytes)
+// apart: device-scope atomics on one address serialise (about 11 ns each on
+// MI355X), so tile t appends its items through counter t % NCLASS into that
+// class's region of the item buffer, and block b folds band minima into word
+// b % NCLASS.
+constexpr uint32_t NCLASS = 64;
+constexpr uint32_t WORD_STRIDE = 64;
 
-// One frontier entry: a cell improved in the previous round, its coordinates
-// and the triangle its key held when the round started.
+// One frontier entry: a cell taken in this round, its coordinates and the
+// triangle its key held when the round started.
 struct FrontierItem {
     uint32_t tri;
     uint32_t x, y, z;
 };
 
-// Marks every seeded cell as changed: round 0 propagates the seeds.
-__global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict__ keys, uint64_t ncells,
-                                                     uint8_t* __restrict__ changed, uint8_t* __restrict__ tile_changed,
-                                                     uint64_t tile) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const bool seeded = keys[i] != KEY_EMPTY;
-        changed[i] = seeded ? 1 : 0;
-        if (i % tile == 0) tile_changed[i / tile] = 1;  // conservatively: every tile is scanned once
-    }
+// Smallest of the NCLASS spread words, computed by every wave on its own.
+__device__ __forceinline__ uint32_t spread_min(const uint32_t* __restrict__ w) {
+    uint32_t m = w[(threadIdx.x & 63u) * WORD_STRIDE];  // NCLASS == 64: one word per lane
+    for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    return m;
 }
 
-// Snapshot of the frontier: every changed cell becomes an item carrying the
-// triangle of its current key, and its flag is cleared. Each block takes a
-// contiguous tile of COMPACT_TILE cells (16 per lane, read as four u32 words),
-// ranks them with a block-wide prefix sum and reserves its output range with a
-// single atomic, so items stay in cell order within a tile and the shared
-// counter sees one atomic per tile. Also resets the counter the next
-// compaction appends to.
+// Flags every seeded cell (round 0 takes the seeds) and folds the smallest
+// seeded distance into the spread minimum words.
+__global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict__ keys, uint64_t ncells,
+                                                     uint8_t* __restrict__ changed, uint8_t* __restrict__ tile_changed,
+                                                     uint64_t tile, uint32_t* __restrict__ min_out) {
+    uint32_t m = UINT32_MAX;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const bool seeded = k != KEY_EMPTY;
+        changed[i] = seeded ? 1 : 0;
+        if (seeded) m = min(m, (uint32_t)(k >> 32));
+        if (i % tile == 0) tile_changed[i / tile] = 1;  // conservatively: every tile is scanned once
+    }
+    block_min_u32(min_out + (blockIdx.x % NCLASS) * WORD_STRIDE, m);
+}
 
+// Snapshot of one round's frontier. The round takes every flagged cell whose
+// distance (the high word of its key, non-negative float bits) is at most
+// min + band, where min is the smallest distance among all flagged cells (the
+// minimum of the spread words min_cur, complete when the round starts); each
+// taken cell becomes an item carrying the triangle of its current key and its
+// flag is cleared. Flagged cells beyond the band keep their flags, and their
+// smallest distance is folded into min_next for the next round.
+//
+// Each block takes a contiguous tile of COMPACT_TILE cells (16 per lane, flags
+// read as one 16-byte vector), ranks the taken cells with a block-wide prefix
+// sum and reserves its output range in its class's region with a single
+// atomic, so items stay in cell order within a tile. Also resets the counters
+// the next compaction appends to.
 __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __restrict__ keys, GridDev g,
                                                           uint8_t* __restrict__ changed,
                                                           uint8_t* __restrict__ tile_changed,
-                                                          FrontierItem* __restrict__ items,
-                                                          uint32_t* __restrict__ count,
-                                                          uint32_t* __restrict__ other_count, int sign) {
-    __shared__ uint32_t wave_tot[4];
+                                                          FrontierItem* __restrict__ items, uint32_t region,
+                                                          uint32_t* __restrict__ ctr_cur,
+                                                          uint32_t* __restrict__ ctr_next,
+                                                          const uint32_t* __restrict__ min_cur,
+                                                          uint32_t* __restrict__ min_next, float band, int sign) {
+    __shared__ uint32_t wave_tot[4], wave_min[4];
     __shared__ uint32_t block_base;
-    const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *other_count = 0u;
+    const uint32_t ncells = g.n[0] * g.n[1] * g.n[2];  // below 2^32 (checked on the host)
+    if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
+    const float mf = bitsf(spread_min(min_cur));
+    float limit = mf + band;
+    if (!(limit >= mf)) limit = INFINITY;  // a NaN band (or no flagged cell) takes every flagged cell
+    const uint32_t lbits = fbits(limit);  // non-negative floats order as their bits
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    for (uint64_t tile = (uint64_t)blockIdx.x * COMPACT_TILE; tile < ncells;
-         tile += (uint64_t)gridDim.x * COMPACT_TILE) {
-        // tiles no cell of which improved in the last round hold no frontier cell
-        const uint64_t tile_id = tile / COMPACT_TILE;
-        if (!tile_changed[tile_id]) continue;  // block-uniform
-        __syncthreads();  // every lane has read the flag before it is cleared
-        if (threadIdx.x == 0) tile_changed[tile_id] = 0;
-        const uint64_t c0 = tile + (uint64_t)threadIdx.x * COMPACT_PER_LANE;
-        uint32_t w[COMPACT_PER_LANE / 4];
-        if (c0 + COMPACT_PER_LANE <= ncells) {
-            const uint4 v = *reinterpret_cast<const uint4*>(changed + c0);  // c0 is a multiple of 16
-            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
-        } else {
+    for (uint32_t tile = blockIdx.x * (uint32_t)COMPACT_TILE; tile < ncells; tile += gridDim.x * (uint32_t)COMPACT_TILE) {
+        const uint32_t tile_id = tile / COMPACT_TILE;
+        const uint32_t c0 = tile + threadIdx.x * COMPACT_PER_LANE;
+        // the tile flag and this lane's 16 cell flags are read together
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        const uint8_t tflag = tile_changed[tile_id];
+        if (c0 < ncells) v = *reinterpret_cast<const uint4*>(changed + c0);  // allocation padded to 16 bytes
+        if (!tflag) continue;  // block-uniform: no flagged cell in the tile
+        uint32_t flagged = 0;  // bit k: cell c0 + k is flagged
+        {
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int k = 0; k < COMPACT_PER_LANE / 4; ++k) {
-                uint32_t x = 0;
-                for (int b = 0; b < 4; ++b) {
-                    uint64_t c = c0 + 4 * k + b;
-                    if (c < ncells && changed[c]) x |= 1u << (8 * b);
-                }
-                w[k] = x;
-            }
+            for (int k = 0; k < COMPACT_PER_LANE; ++k) flagged |= ((w[k >> 2] >> (8 * (k & 3))) & 1u) << k;
+            if (c0 >= ncells) flagged = 0;
+            else if (ncells - c0 < (uint32_t)COMPACT_PER_LANE) flagged &= (1u << (ncells - c0)) - 1u;
         }
-        uint32_t mine = 0;
+        // every flagged cell's key first, so the loads are in flight together
+        uint64_t kv[COMPACT_PER_LANE];
 #pragma unroll
-        for (int k = 0; k < COMPACT_PER_LANE / 4; ++k) mine += __popc(w[k] & 0x01010101u);
-        // block-wide exclusive prefix of `mine`
+        for (int k = 0; k < COMPACT_PER_LANE; ++k) kv[k] = ((flagged >> k) & 1u) ? keys[c0 + k] : KEY_EMPTY;
+        uint32_t take = 0, dmin = UINT32_MAX;
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER_LANE; ++k) {
+            if (!((flagged >> k) & 1u)) continue;
+            const uint32_t hi = (uint32_t)(kv[k] >> 32);
+            if (hi <= lbits) take |= 1u << k;
+            else dmin = min(dmin, hi);
+        }
+        const uint32_t deferred = flagged & ~take;
+        const uint32_t mine = __popc(take);
+        // block-wide exclusive prefix of `mine`, block minimum of `dmin`
         uint32_t x = mine;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             uint32_t y = __shfl_up(x, off, 64);
             if ((int)lane >= off) x += y;
         }
+        uint32_t wm = dmin;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) wm = min(wm, (uint32_t)__shfl_xor((int)wm, off, 64));
         if (lane == 63) wave_tot[wave] = x;
+        if (lane == 0) wave_min[wave] = wm;
         __syncthreads();
-        uint32_t before = 0, total = 0;
+        uint32_t before = 0, total = 0, bmin = UINT32_MAX;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             before += (uint32_t)q < wave ? wave_tot[q] : 0u;
             total += wave_tot[q];
+            bmin = min(bmin, wave_min[q]);
         }
-        if (threadIdx.x == 0) block_base = total ? atomicAdd(count, total) : 0u;
+        const uint32_t cls = tile_id % NCLASS;
+        if (threadIdx.x == 0) block_base = total ? atomicAdd(ctr_cur + cls * WORD_STRIDE, total) : 0u;
         __syncthreads();
-        uint32_t slot = block_base + before + x - mine;
-        if (mine) {
-#pragma unroll
-            for (int k = 0; k < COMPACT_PER_LANE / 4; ++k) {
-                if (!(w[k] & 0x01010101u)) continue;
-                for (int b = 0; b < 4; ++b) {
-                    if (!((w[k] >> (8 * b)) & 1u)) continue;
-                    const uint64_t i = c0 + 4 * k + b;
-                    const uint32_t z = (uint32_t)(i % g.n[2]);
-                    const uint64_t r = i / g.n[2];
-                    const uint32_t y = (uint32_t)(r % g.n[1]);
-                    const uint32_t xx = (uint32_t)(r / g.n[1]);
-                    items[slot++] = FrontierItem{key_tri(keys[i], sign), xx, y, z};
-                }
-            }
-            if (c0 + COMPACT_PER_LANE <= ncells) {
-                *reinterpret_cast<uint4*>(changed + c0) = make_uint4(0u, 0u, 0u, 0u);
-            } else {
-                for (uint64_t c = c0; c < ncells; ++c) changed[c] = 0;
-            }
+        if (threadIdx.x == 0) {
+            tile_changed[tile_id] = bmin != UINT32_MAX ? 1 : 0;  // deferred cells keep the tile flagged
+            if (bmin != UINT32_MAX) atomicMin(min_next + cls * WORD_STRIDE, bmin);  // nothing waits for it
         }
-        __syncthreads();  // wave_tot / block_base are reused by the next tile
+        if (take) {
+            uint32_t slot = cls * region + block_base + before + x - mine;
+#pragma unroll
+            for (int k = 0; k < COMPACT_PER_LANE; ++k) {
+                if (!((take >> k) & 1u)) continue;
+                const uint32_t i = c0 + k;
+                const uint32_t z = i % g.n[2];
+                const uint32_t r = i / g.n[2];
+                items[slot++] = FrontierItem{key_tri(kv[k], sign), r / g.n[1], r % g.n[1], z};
+            }
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int k = 0; k < COMPACT_PER_LANE; ++k) w[k >> 2] |= ((deferred >> k) & 1u) << (8 * (k & 3));
+            *reinterpret_cast<uint4*>(changed + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        __syncthreads();  // wave_tot / wave_min / block_base are reused by the next tile
     }
 }
 
 // Tries each item's triangle on the item's 26 neighbours (one lane per pair);
-// a strictly smaller key is stored with atomicMin and flags the cell for the
-// next round. The read before the atomic is a plain (cacheable) load: keys only
-// decrease, so a stale value can only cause an unneeded atomic.
+// a strictly smaller key is stored with atomicMin and flags the cell for a later
+// round. The read before the atomic is a plain (cacheable) load: keys only
+// decrease, so a stale value can only cause an unneeded atomic. The smallest
+// improved distance is folded into min_next; min_cur (read by this round's
+// compaction, written by the next one) is reset to "none". The frontier size
+// goes to *total_out for the host.
 template <int SIGN>
-__global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items,
-                                                            const uint32_t* __restrict__ count,
+__global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
+                                                            const uint32_t* __restrict__ ctr_cur,
                                                             const TriRec* __restrict__ tris, GridDev g,
                                                             uint64_t* keys, uint8_t* __restrict__ changed,
-                                                            uint8_t* __restrict__ tile_changed, DevFlags* flags,
+                                                            uint8_t* __restrict__ tile_changed,
+                                                            uint32_t* __restrict__ min_next,
+                                                            uint32_t* __restrict__ min_cur,
+                                                            uint32_t* __restrict__ total_out, DevFlags* flags,
                                                             unsigned long long* evals) {
-    const uint64_t work = (uint64_t)(*count) * 26u;
+    // prefix[c]: items in the classes before c
+    __shared__ uint32_t prefix[NCLASS + 1];
+    if (threadIdx.x < 64u) {
+        const uint32_t c = ctr_cur[threadIdx.x * WORD_STRIDE];
+        uint32_t x = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(x, off, 64);
+            if ((int)threadIdx.x >= off) x += y;
+        }
+        prefix[threadIdx.x + 1] = x;
+        if (threadIdx.x == 0) prefix[0] = 0;
+        if (blockIdx.x == 0) {
+            min_cur[threadIdx.x * WORD_STRIDE] = UINT32_MAX;
+            if (threadIdx.x == 63) *total_out = x;
+        }
+    }
+    __syncthreads();
+    const uint32_t work = prefix[NCLASS] * 26u;  // below 2^32: the frontier holds fewer than 2^32 / 26 cells
     bool nan_seen = false;
-    uint32_t n_evals = 0;
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < work;
-         j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t jj = (uint32_t)j;  // work < 2^32: at most 2^32 / 26 frontier cells
-        const uint32_t f = jj / 26u;
-        uint32_t k = jj - f * 26u;
+    uint32_t n_evals = 0, imin = UINT32_MAX;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < work; j += gridDim.x * blockDim.x) {
+        const uint32_t f = j / 26u;
+        uint32_t k = j - f * 26u;
         k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
-        const int dx = (int)(k / 9u) - 1, dy = (int)((k / 3u) % 3u) - 1, dz = (int)(k % 3u) - 1;
-        const FrontierItem it = items[f];
-        const int64_t x = (int64_t)it.x + dx, y = (int64_t)it.y + dy, z = (int64_t)it.z + dz;
-        if (x < 0 || y < 0 || z < 0 || x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
-        const uint64_t ci = cell_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+        uint32_t cls = 0;  // the last class whose prefix is <= f (a non-empty one)
+#pragma unroll
+        for (uint32_t step = NCLASS / 2; step; step >>= 1)
+            if (prefix[cls + step] <= f) cls += step;
+        const FrontierItem it = items[cls * region + (f - prefix[cls])];
+        const uint32_t x = it.x + k / 9u - 1u, y = it.y + (k / 3u) % 3u - 1u, z = it.z + k % 3u - 1u;
+        if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;  // -1 wraps to 2^32 - 1
+        const uint32_t ci = (x * g.n[1] + y) * g.n[2] + z;  // grid.rs:276-278
         const uint64_t cur = keys[ci];
         // A neighbour already holding this triangle holds exactly the key the
         // triangle would give it, and keys only decrease: nothing to try.
         if (cur != KEY_EMPTY && key_tri(cur, SIGN) == it.tri) continue;
         const TriRec tr = tris[it.tri];
-        float d = cell_dist<SIGN>(g, (uint32_t)x, (uint32_t)y, (uint32_t)z, tr);
+        float d = cell_dist<SIGN>(g, x, y, z, tr);
         n_evals++;
         nan_seen |= d != d;
         if (!key_valid(d)) continue;
@@ -335,9 +401,11 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
         if (key < old) {
             changed[ci] = 1;
             tile_changed[ci / COMPACT_TILE] = 1;
+            imin = min(imin, (uint32_t)(key >> 32));
         }
     }
     if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+    block_min_u32(min_next + (blockIdx.x % NCLASS) * WORD_STRIDE, imin);
     if (evals) block_count(evals, n_evals);
 }
 
@@ -517,22 +585,37 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     SDFX_HIP(hipGetLastError());
 
     // 2. frontier rounds
-    if (ncells >= (1ull << 32) / 26) throw Error(SDFX_ERR_INVALID_ARGUMENT, "grid too large for one frontier");
+    // 32-bit cell and work indices: 26 neighbour slots per cell plus one grid
+    // stride stay below 2^32
+    if (ncells * 26 + (1ull << 24) >= (1ull << 32))
+        throw Error(SDFX_ERR_INVALID_ARGUMENT, "grid too large for one frontier (at most 165,000,000 cells)");
     const uint64_t ntiles = (ncells + COMPACT_TILE - 1) / COMPACT_TILE;
-    uint32_t* cnt = A.alloc<uint32_t>(2);
+    // per parity of the round: NCLASS item counters, NCLASS band minima; then
+    // the two frontier totals
+    const uint64_t spread = (uint64_t)NCLASS * WORD_STRIDE;
+    uint32_t* words = A.alloc<uint32_t>(4 * spread + 2);
+    uint32_t* ctr[2] = {words, words + spread};
+    uint32_t* minw[2] = {words + 2 * spread, words + 3 * spread};
+    uint32_t* totals = words + 4 * spread;
+    const uint32_t region = (uint32_t)((ntiles + NCLASS - 1) / NCLASS * COMPACT_TILE);  // items per class
     uint8_t* changed = A.alloc<uint8_t>((ncells + 15) & ~15ull);  // 16-byte vector reads
     uint8_t* tile_changed = A.alloc<uint8_t>(ntiles);
-    FrontierItem* items = A.alloc<FrontierItem>(ncells);
-    fill_u32(cnt, 2, 0u, s);
+    FrontierItem* items = A.alloc<FrontierItem>((uint64_t)region * NCLASS);
+    fill_u32(ctr[0], 2 * spread, 0u, s);
+    fill_u32(minw[0], 2 * spread, UINT32_MAX, s);
+    fill_u32(totals, 2, 0u, s);
     SDFX_LAUNCH(k_mark_seeded, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, changed, tile_changed,
-                (uint64_t)COMPACT_TILE);
+                (uint64_t)COMPACT_TILE, minw[0]);
     SDFX_HIP(hipGetLastError());
+    const float band =
+        GRID_BAND_CELLS * std::fmax(std::fmax(std::fabs(g.size[0]), std::fabs(g.size[1])), std::fabs(g.size[2]));
     const unsigned prop_blocks = 4096;
     const unsigned compact_blocks = (unsigned)std::min<uint64_t>(ntiles, 4096);
     const int batch = 8;
-    // Every round strictly lowers some key, so the loop ends; the cap only turns
-    // an inconsistency into an error instead of a hang.
-    const uint64_t max_rounds = 64ull * (g.n[0] + g.n[1] + g.n[2]) + 1024;
+    // Every round takes at least the nearest flagged cell and every improvement
+    // strictly lowers a key, so the loop ends; the cap only turns an
+    // inconsistency into an error instead of a hang.
+    const uint64_t max_rounds = 256ull * (g.n[0] + g.n[1] + g.n[2]) + 4096;
     // Batches are enqueued one ahead of the one being checked: while the host
     // waits for batch k's frontier size, batch k+1 is already on the GPU.
     BatchProbe probe;
@@ -541,13 +624,14 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     auto enqueue_batch = [&](int slot) {
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact, dim3(compact_blocks), dim3(256), 0, s, keys, g, changed, tile_changed,
-                        items, cnt + cur, cnt + (cur ^ 1), (int)SIGN);
-            SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, cnt + cur, d_tris, g,
-                        keys, changed, tile_changed, d_flags, evals ? evals + seed_slots : nullptr);
+                        items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1], band, (int)SIGN);
+            SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
+                        d_tris, g, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], totals + cur, d_flags,
+                        evals ? evals + seed_slots : nullptr);
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());
-        probe.record(slot, cnt + (cur ^ 1), s);  // size of the batch's last frontier
+        probe.record(slot, totals + (cur ^ 1), s);  // size of the batch's last frontier
     };
     enqueue_batch(0);
     for (int slot = 0;; slot ^= 1) {

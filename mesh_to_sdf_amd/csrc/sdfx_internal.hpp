@@ -113,6 +113,26 @@ __device__ __forceinline__ void block_count(unsigned long long* out, uint32_t v)
     __syncthreads();
     if (threadIdx.x == 0 && sdfx_block_sum) out[blockIdx.x] += sdfx_block_sum;
 }
+// Folds m into *out with atomicMin unless *out already holds a value <= m. The
+// words only decrease, so a stale read can only cost an unneeded atomic; the
+// read keeps thousands of blocks from queueing atomics on one address.
+__device__ __forceinline__ void atomic_min_if_lower(uint32_t* out, uint32_t m) {
+    if (m < __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(out, m);
+}
+// Block-wide minimum of v folded into *out (nothing when every lane holds
+// UINT32_MAX). All lanes of the block must call it.
+__device__ __forceinline__ void block_min_u32(uint32_t* out, uint32_t v) {
+    __shared__ uint32_t sdfx_wave_min[16];
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    if ((threadIdx.x & 63u) == 0) sdfx_wave_min[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = UINT32_MAX;
+        for (uint32_t w = 0; w < (blockDim.x + 63u) / 64u; ++w) m = min(m, sdfx_wave_min[w]);
+        if (m != UINT32_MAX) atomic_min_if_lower(out, m);
+    }
+    __syncthreads();
+}
 // Sums `n` per-block counter slots on the host (synchronises the stream).
 uint64_t sum_block_counters(const unsigned long long* d_slots, uint64_t n, hipStream_t stream);
 

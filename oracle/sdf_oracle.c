@@ -1060,18 +1060,31 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
     if (seed_out)
         for (uint64_t i = 0; i < ncells; ++i) seed_out[i] = key_value(keys[i], sign);
     for (uint64_t i = 0; i < ncells; ++i) changed[i] = keys[i] != KEY_EMPTY;
+    const float band = ORC_GRID_BAND_CELLS * fmaxf(fmaxf(fabsf(g->size.x), fabsf(g->size.y)), fabsf(g->size.z));
     int64_t rounds = 0;
     uint64_t n_evals = 0;
     for (;;) {
-        uint64_t nf = 0;
+        /* smallest |distance| among the flagged cells (keys order as these bits) */
+        uint32_t m = UINT32_MAX;
         for (uint64_t i = 0; i < ncells; ++i)
-            if (changed[i]) {
-                changed[i] = 0;
-                fr_cell[nf] = i;
-                fr_tri[nf] = key_tri(keys[i], sign);
-                nf++;
-            }
-        if (!nf) break;
+            if (changed[i] && (uint32_t)(keys[i] >> 32) < m) m = (uint32_t)(keys[i] >> 32);
+        if (m == UINT32_MAX) break;
+        float mf;
+        memcpy(&mf, &m, 4);
+        float limit = mf + band;
+        if (!(limit >= mf)) limit = INFINITY; /* a NaN band takes every flagged cell */
+        uint64_t nf = 0;
+        for (uint64_t i = 0; i < ncells; ++i) {
+            if (!changed[i]) continue;
+            uint32_t b = (uint32_t)(keys[i] >> 32);
+            float v;
+            memcpy(&v, &b, 4);
+            if (!(v <= limit)) continue;
+            changed[i] = 0;
+            fr_cell[nf] = i;
+            fr_tri[nf] = key_tri(keys[i], sign);
+            nf++;
+        }
         rounds++;
 #pragma omp parallel for num_threads(n_threads) schedule(dynamic, 256) reduction(| : nan_seen) reduction(+ : n_evals)
         for (int64_t f = 0; f < (int64_t)nf; ++f) {

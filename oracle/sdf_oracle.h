@@ -92,11 +92,19 @@ typedef struct {
  * reference does (result depends on timing).
  * schedule ORC_GRID_FRONTIER: not one of the reference's schedules. The same
  * relaxation rule (a cell's triangle is tried on its 27 neighbours, kept on
- * strict improvement) run as rounds over the frontier of cells improved in the
- * previous round, each cell holding the minimum (distance bits, triangle index)
- * key. It is deterministic and is the schedule the device pipeline runs, so it
- * pins the device kernels bit-exactly; its distance from the reference's own
- * schedules is measured separately. */
+ * strict improvement) run as rounds, each cell holding the minimum (distance
+ * bits, triangle index) key. A cell is flagged when its key improves (seeded
+ * cells start flagged). Round r takes the flagged cells whose distance is at
+ * most T_r = m_r + band, where m_r is the smallest distance of any flagged cell
+ * and band = ORC_GRID_BAND_CELLS * max|cell_size| (both in FP32), snapshots
+ * their (cell, triangle), clears their flags and tries each on its neighbours;
+ * the other flagged cells wait. Taking the nearest cells first, like the
+ * reference's heap, keeps a cell from being improved many times by far
+ * triangles before its near ones arrive. It is deterministic and is the
+ * schedule the device pipeline runs, so it pins the device kernels
+ * bit-exactly; its distance from the reference's own schedules is measured
+ * separately. */
+#define ORC_GRID_BAND_CELLS 3.0f
 int orc_generate_grid_sdf(const float* vertices, uint64_t n_vertices, const uint32_t* indices, uint64_t n_indices,
                           int topology, const float* first_cell, const float* cell_size,
                           const uint64_t* cell_count, int sign, float* out, int n_heaps, int schedule,
