@@ -38,11 +38,46 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the HIP-event kernel-timing pass (for runs under an external profiler)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (the reference for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """Three copies of 256 MiB (read 256 MiB, write 256 MiB each): under
+    `rocprofv3 --pmc FETCH_SIZE` (or WRITE_SIZE) they give the counter's reading
+    for a known number of HBM bytes."""
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copies": 3, "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic.json: mean FETCH_SIZE / WRITE_SIZE KiB per dispatch, each
+    scaled by the reading the same counter gives for a copy of known size), or
+    None when there is no entry for it."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh).get(kernel)
+    if not t:
+        return None
+    return 1024.0 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] + t["write_kib_per_dispatch"] * t["write_scale"])
 
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every GPU machine here has one device; "
                                    "run with --gpus 1"}))
@@ -129,7 +164,14 @@ def main():
         "lbvh.vote_nodes", "lbvh.vote_leaves", "lbvh.queries")}
     _lib.profile_enable(False)
     kernels = sorted(((name, t, n) for name, (t, n) in prof.items()), key=lambda x: -x[1])
-    dom_name, dom_ms, dom_launches = kernels[0]
+    dom_name = kernels[0][0]
+    # The dominant kernel's average duration: the same K steps once more with an
+    # event pair on that kernel's launches only and the work counters off.
+    _lib.profile_enable(True)
+    _lib.profile_select(dom_name, counters=False)
+    timed(args.steps)
+    dom_ms, dom_launches = _lib.profile_read()[dom_name]
+    _lib.profile_enable(False)
     mesh_bytes = 12 * len(v) + 12 * len(f)
     if args.workload == "grid":
         # 80 FP32 operations per point-triangle distance (geo.rs:26-30, 70-138 as written)
@@ -142,14 +184,14 @@ def main():
             flops_per_launch = 80.0 * evals / dom_launches
             achieved = flops_per_launch / (dom_ms / dom_launches / 1e3) / 1e12
             roofline = {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
-                        "frac": achieved / 157.3, "traffic": None, "kernel": dom_name,
+                        "frac": achieved / 157.3, "traffic": measured_traffic(dom_name), "kernel": dom_name,
                         "engine": "FP32 VALU (no MFMA: 157.3 TFLOP/s is MI355X's FP32 vector peak)",
                         "work_per_launch": f"{evals / dom_launches:.0f} distance evaluations x 80 FLOP",
                         "avg_launch_ms": dom_ms / dom_launches}
         else:
             achieved = (4.0 * samples + mesh_bytes) / (dom_ms / dom_launches / 1e3) / 1e9
             roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                        "frac": achieved / 8000.0, "traffic": None, "kernel": dom_name}
+                        "frac": achieved / 8000.0, "traffic": measured_traffic(dom_name), "kernel": dom_name}
         step_bytes = 4.0 * samples + mesh_bytes
         roofline_hbm_step = {"bytes_per_step": step_bytes, "GB/s": step_bytes / (ms / 1e3) / 1e9,
                              "frac": step_bytes / (ms / 1e3) / 1e9 / 8000.0,
@@ -158,7 +200,7 @@ def main():
         per_launch = 16.0 * samples + mesh_bytes  # 12 B query + 4 B result per sample, mesh once
         achieved = per_launch / (dom_ms / dom_launches / 1e3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                    "frac": achieved / 8000.0, "traffic": None, "kernel": dom_name,
+                    "frac": achieved / 8000.0, "traffic": measured_traffic(dom_name), "kernel": dom_name,
                     "avg_launch_ms": dom_ms / dom_launches}
         roofline_hbm_step = None
 

@@ -160,6 +160,11 @@ void sdfx_profile_enable(int on);
 int sdfx_profile_read(int max_entries, char* names, double* total_ms, uint64_t* launches);
 /* Work counters accumulated while profiling (e.g. "grid.propagate_distance_evals"). */
 uint64_t sdfx_profile_counter(const char* name);
+/* Narrows the profiling started by sdfx_profile_enable(1): only kernels whose
+ * name starts with kernel_prefix get an event pair ("" or NULL: all), and work
+ * counters run only when counters != 0. Fewer event pairs disturb the launch
+ * stream less, so one kernel's average duration is closest to a profiler's. */
+void sdfx_profile_select(const char* kernel_prefix, int counters);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ SIGNATURES = {
     "sdfx_profile_enable": (None, [_i]),
     "sdfx_profile_read": (_i, [_i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _u64]),
     "sdfx_profile_counter": (ctypes.c_uint64, [ctypes.c_char_p]),
+    "sdfx_profile_select": (None, [ctypes.c_char_p, _i]),
     "sdfx_set_tuning": (_i, [ctypes.c_char_p, ctypes.c_int64]),
     "sdfx_get_tuning": (ctypes.c_int64, [ctypes.c_char_p]),
 }
@@ -100,6 +101,10 @@ def profile_read(max_entries: int = 128) -> dict:
         name = names.raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
         out[name] = (ms[i], int(cnt[i]))
     return out
+
+
+def profile_select(kernel_prefix: str = "", counters: bool = True) -> None:
+    lib().sdfx_profile_select(kernel_prefix.encode(), 1 if counters else 0)
 
 
 def profile_counter(name: str) -> int:

@@ -193,9 +193,10 @@ constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
 constexpr float GRID_BAND_CELLS = 3.0f;
 // Round bookkeeping is spread over NCLASS words, WORD_STRIDE u32 (256 bytes)
 // apart: device-scope atomics on one address serialise (about 11 ns each on
-// MI355X), so tile t appends its items through counter t % NCLASS into that
-// class's region of the item buffer, and block b folds band minima into word
-// b % NCLASS.
+// MI355X), so tile t appends its items through the counter of its class
+// t * NCLASS / ntiles (a run of neighbouring tiles) into that class's region of
+// the item buffer, and block b folds band minima into word b % NCLASS. The
+// item buffer thus stays in tile order, i.e. in spatial order.
 constexpr uint32_t NCLASS = 64;
 constexpr uint32_t WORD_STRIDE = 64;
 
@@ -254,6 +255,7 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
     __shared__ uint32_t wave_tot[4], wave_min[4];
     __shared__ uint32_t block_base;
     const uint32_t ncells = g.n[0] * g.n[1] * g.n[2];  // below 2^32 (checked on the host)
+    const uint32_t ntiles = (ncells + COMPACT_TILE - 1) / COMPACT_TILE;
     if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
     const float mf = bitsf(spread_min(min_cur));
     float limit = mf + band;
@@ -310,7 +312,7 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
             total += wave_tot[q];
             bmin = min(bmin, wave_min[q]);
         }
-        const uint32_t cls = tile_id % NCLASS;
+        const uint32_t cls = (uint32_t)((uint64_t)tile_id * NCLASS / ntiles);
         if (threadIdx.x == 0) block_base = total ? atomicAdd(ctr_cur + cls * WORD_STRIDE, total) : 0u;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -372,9 +374,15 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
     }
     __syncthreads();
     const uint32_t work = prefix[NCLASS] * 26u;  // below 2^32: the frontier holds fewer than 2^32 / 26 cells
+    // Each XCD (blocks b with b % XCDS == x) takes one contiguous eighth of the
+    // items, which lie in spatial order, so keys and triangles of neighbouring
+    // cells share that XCD's L2.
+    const uint32_t chunk = (work + XCDS - 1) / XCDS;
+    const uint32_t j0 = (blockIdx.x % XCDS) * chunk, j1 = min(work, j0 + chunk);
+    const uint32_t stride = gridDim.x / XCDS * blockDim.x;
     bool nan_seen = false;
     uint32_t n_evals = 0, imin = UINT32_MAX;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < work; j += gridDim.x * blockDim.x) {
+    for (uint32_t j = j0 + blockIdx.x / XCDS * blockDim.x + threadIdx.x; j < j1; j += stride) {
         const uint32_t f = j / 26u;
         uint32_t k = j - f * 26u;
         k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)

@@ -609,7 +609,8 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
                                                     uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                     const uint32_t* __restrict__ order, float* __restrict__ out,
                                                     DevFlags* flags, unsigned long long* stats) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // with a Morton order, each XCD walks one contiguous stretch of it
+    uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = i < nq;  // no early return: the statistics below use block barriers
     TravStats ts;
     if (active) {
@@ -680,7 +681,7 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     if (!nq || !bvh.n) return;
     hipStream_t s = A.stream;
     const uint32_t* order = query_order(bvh, d_q, nq, A);
-    unsigned blocks = blocks_for(nq, 256);
+    const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
     unsigned long long* stats = nullptr;
     if (profile_enabled() && mode != QM_NORMAL_FOLD) {
         stats = A.alloc<unsigned long long>(4ull * blocks);
@@ -719,7 +720,7 @@ __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restri
                                                          uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                          const uint32_t* __restrict__ order, float* __restrict__ vec,
                                                          uint8_t* __restrict__ inside, DevFlags* flags) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     if (order) i = order[i];
     float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
@@ -746,7 +747,8 @@ void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float*
     if (!nq || !bvh.n) return;
     hipStream_t s = A.stream;
     const uint32_t* order = query_order(bvh, d_q, nq, A);
-    SDFX_LAUNCH(k_closest_vectors, dim3(blocks_for(nq, 256)), dim3(256), 0, s, bvh.nodes.get(),
+    const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
+    SDFX_LAUNCH(k_closest_vectors, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
                        bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_vec, d_inside, d_flags);
     SDFX_HIP(hipGetLastError());
 }

@@ -1,6 +1,8 @@
 // sdfx_runtime.cpp — error state, per-thread streams, caching allocator.
 #include "sdfx_runtime.hpp"
 
+#include <cstring>
+
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -135,6 +137,8 @@ struct ProfRecord {
 struct ProfState {
     std::mutex mu;
     bool enabled = false;
+    bool counters_on = true;  // work counters (distance evaluations, node visits)
+    std::string select;       // only kernels whose name starts with this get events ("" = all)
     std::vector<ProfRecord> pending;
     std::vector<hipEvent_t> free_events;
     std::map<std::string, std::pair<double, uint64_t>> totals;  // name -> (ms, launches)
@@ -171,7 +175,10 @@ void resolve(ProfState& P) {
 }
 }  // namespace
 
-bool profile_enabled() { return prof().enabled; }
+bool profile_enabled() {
+    ProfState& P = prof();
+    return P.enabled && P.counters_on;
+}
 
 void profile_count(const char* name, uint64_t value) {
     ProfState& P = prof();
@@ -185,6 +192,7 @@ ProfPair prof_begin(const char* name) {
     ProfState& P = prof();
     if (!P.enabled) return p;
     std::lock_guard<std::mutex> lk(P.mu);
+    if (!P.select.empty() && std::strncmp(name, P.select.c_str(), P.select.size()) != 0) return p;
     p.a = take_event(P);
     p.b = take_event(P);
     if (!p.a || !p.b) {
@@ -215,6 +223,15 @@ void sdfx_profile_enable(int on) {
     P.totals.clear();
     P.counters.clear();
     P.enabled = on != 0;
+    P.counters_on = true;
+    P.select.clear();
+}
+
+void sdfx_profile_select(const char* kernel_prefix, int counters) {
+    auto& P = sdfx::prof();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.select = kernel_prefix ? kernel_prefix : "";
+    P.counters_on = counters != 0;
 }
 
 int sdfx_set_tuning(const char* key, int64_t value) {
