@@ -357,6 +357,58 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
 constexpr int STACK_DEPTH = 64;  // Karras depth <= 64 (32 code bits + 32 position bits)
 constexpr float PRUNE_SLACK = 1.0f - 1.0f / 65536.0f;
 
+// Traversal stack of one lane: the newest STACK_RING entries live in LDS (a
+// ring in the lane's own column, so lanes never share words and need no
+// barrier), older ones spill to a private array (scratch memory) and return
+// when the ring drains. Entries are two u32 words. The traversal kernels run
+// 256-thread blocks; 8 entries x 8 bytes x 256 lanes = 16 KB of LDS per block
+// keeps 8 waves per SIMD resident.
+constexpr int STACK_RING = 8;
+constexpr int STACK_LANES = 256;
+
+__device__ __forceinline__ uint32_t* stack_lds() {
+    __shared__ uint32_t words[2 * STACK_RING * STACK_LANES];
+    return words;
+}
+
+struct LaneStack {
+    uint32_t* col;  // this lane's column: entry k at col[2 * k * STACK_LANES] / col[(2 * k + 1) * STACK_LANES]
+    uint32_t spill_a[STACK_DEPTH], spill_b[STACK_DEPTH];
+    int head = 0, count = 0, nspill = 0;
+    __device__ LaneStack() : col(stack_lds() + threadIdx.x) {}
+    __device__ __forceinline__ bool empty() const { return count == 0 && nspill == 0; }
+    // false when the stack is full (STACK_DEPTH spilled entries): the entry is dropped
+    __device__ __forceinline__ bool push(uint32_t a, uint32_t b) {
+        if (count == STACK_RING) {  // move the oldest ring entry to the spill array
+            if (nspill == STACK_DEPTH) return false;
+            const int old = head & (STACK_RING - 1);  // head - RING == head (mod RING)
+            spill_a[nspill] = col[2 * old * STACK_LANES];
+            spill_b[nspill] = col[(2 * old + 1) * STACK_LANES];
+            ++nspill;
+            --count;
+        }
+        const int h = head & (STACK_RING - 1);
+        col[2 * h * STACK_LANES] = a;
+        col[(2 * h + 1) * STACK_LANES] = b;
+        ++head;
+        ++count;
+        return true;
+    }
+    __device__ __forceinline__ void pop(uint32_t& a, uint32_t& b) {  // requires !empty()
+        if (count > 0) {
+            --head;
+            --count;
+            const int h = head & (STACK_RING - 1);
+            a = col[2 * h * STACK_LANES];
+            b = col[(2 * h + 1) * STACK_LANES];
+        } else {
+            --nspill;
+            a = spill_a[nspill];
+            b = spill_b[nspill];
+        }
+    }
+};
+
 struct Nearest {
     float d2;
     uint32_t index;  // topology index (tie-break)
@@ -388,9 +440,7 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
         visit_leaf_nearest(leaves, ~root, px, py, pz, best, ts);
         return best;
     }
-    int32_t stack_node[STACK_DEPTH];
-    float stack_d[STACK_DEPTH];
-    int sp = 0;
+    LaneStack st;
     int32_t node = root;
     for (uint32_t visits = 0;; ++visits) {
         if (visits > max_visits) {  // a tree visits each node once; more means a malformed tree
@@ -414,13 +464,7 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
         }
         if (gl && gr) {
             bool left_first = dl <= dr;
-            if (sp < STACK_DEPTH) {
-                stack_node[sp] = left_first ? nd.right : nd.left;
-                stack_d[sp] = left_first ? dr : dl;
-                sp++;
-            } else {
-                overflow = true;
-            }
+            if (!st.push((uint32_t)(left_first ? nd.right : nd.left), fbits(left_first ? dr : dl))) overflow = true;
             node = left_first ? nd.left : nd.right;
             continue;
         }
@@ -433,10 +477,11 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
             continue;
         }
         bool found = false;
-        while (sp > 0) {
-            --sp;
-            if (stack_d[sp] * PRUNE_SLACK <= best.d2) {
-                node = stack_node[sp];
+        while (!st.empty()) {
+            uint32_t a, b;
+            st.pop(a, b);
+            if (bitsf(b) * PRUNE_SLACK <= best.d2) {
+                node = (int32_t)a;
                 found = true;
                 break;
             }
@@ -468,9 +513,7 @@ __device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec
         visit(~root);
         return acc;
     }
-    int32_t stack_node[STACK_DEPTH];
-    float stack_d[STACK_DEPTH];
-    int sp = 0;
+    LaneStack st;
     int32_t node = root;
     for (uint32_t visits = 0;; ++visits) {
         if (visits > max_visits) {  // a tree visits each node once; more means a malformed tree
@@ -493,13 +536,7 @@ __device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec
         }
         if (gl && gr) {
             bool left_first = dl <= dr;
-            if (sp < STACK_DEPTH) {
-                stack_node[sp] = left_first ? nd.right : nd.left;
-                stack_d[sp] = left_first ? dr : dl;
-                sp++;
-            } else {
-                overflow = true;
-            }
+            if (!st.push((uint32_t)(left_first ? nd.right : nd.left), fbits(left_first ? dr : dl))) overflow = true;
             node = left_first ? nd.left : nd.right;
             continue;
         }
@@ -512,10 +549,11 @@ __device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec
             continue;
         }
         bool found = false;
-        while (sp > 0) {
-            --sp;
-            if (stack_d[sp] <= bound2()) {
-                node = stack_node[sp];
+        while (!st.empty()) {
+            uint32_t a, b;
+            st.pop(a, b);
+            if (bitsf(b) <= bound2()) {
+                node = (int32_t)a;
                 found = true;
                 break;
             }
@@ -555,8 +593,7 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
     if (root < 0) {
         ray_leaf(leaves, ~root, 7u, px, py, pz, cnt, ts);
     } else {
-        uint32_t stack[STACK_DEPTH];
-        int sp = 0;
+        LaneStack st;
         int32_t node = root;
         uint32_t axes = 7u;
         for (uint32_t visits = 0;; ++visits) {
@@ -577,8 +614,7 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
                 ar = 0;
             }
             if (al && ar) {
-                if (sp < STACK_DEPTH) stack[sp++] = ((uint32_t)nd.right << 3) | ar;
-                else overflow = true;
+                if (!st.push((uint32_t)nd.right, ar)) overflow = true;
                 node = nd.left;
                 axes = al;
                 continue;
@@ -593,10 +629,11 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
                 axes = ar;
                 continue;
             }
-            if (sp == 0) break;
-            uint32_t e = stack[--sp];
-            node = (int32_t)(e >> 3);
-            axes = e & 7u;
+            if (st.empty()) break;
+            uint32_t a, b;
+            st.pop(a, b);
+            node = (int32_t)a;
+            axes = b;
         }
     }
     int insides = (int)(cnt[0] & 1u) + (int)(cnt[1] & 1u) + (int)(cnt[2] & 1u);
