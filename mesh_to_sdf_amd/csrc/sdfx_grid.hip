@@ -191,6 +191,11 @@ constexpr int COMPACT_PER_LANE = 16;
 constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
 // Width of the distance band a round takes, in units of the largest |cell size|.
 constexpr float GRID_BAND_CELLS = 3.0f;
+// A round after one that took fewer than max(GRID_RELEASE_MIN, cells /
+// GRID_RELEASE_DIV) cells takes every flagged cell: once the front is thin,
+// waiting for the band only adds rounds.
+constexpr uint64_t GRID_RELEASE_MIN = 1024;
+constexpr uint64_t GRID_RELEASE_DIV = 128;
 // Round bookkeeping is spread over NCLASS words, WORD_STRIDE u32 (256 bytes)
 // apart: device-scope atomics on one address serialise (about 11 ns each on
 // MI355X), so tile t appends its items through the counter of its class
@@ -234,10 +239,12 @@ __global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict_
 // Snapshot of one round's frontier. The round takes every flagged cell whose
 // distance (the high word of its key, non-negative float bits) is at most
 // min + band, where min is the smallest distance among all flagged cells (the
-// minimum of the spread words min_cur, complete when the round starts); each
-// taken cell becomes an item carrying the triangle of its current key and its
-// flag is cleared. Flagged cells beyond the band keep their flags, and their
-// smallest distance is folded into min_next for the next round.
+// minimum of the spread words min_cur, complete when the round starts), or
+// every flagged cell when the previous round took fewer than `release` cells
+// (*prev_taken); each taken cell becomes an item carrying the triangle of its
+// current key and its flag is cleared. Flagged cells beyond the band keep
+// their flags, and their smallest distance is folded into min_next for the
+// next round.
 //
 // Each block takes a contiguous tile of COMPACT_TILE cells (16 per lane, flags
 // read as one 16-byte vector), ranks the taken cells with a block-wide prefix
@@ -251,7 +258,9 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
                                                           uint32_t* __restrict__ ctr_cur,
                                                           uint32_t* __restrict__ ctr_next,
                                                           const uint32_t* __restrict__ min_cur,
-                                                          uint32_t* __restrict__ min_next, float band, int sign) {
+                                                          uint32_t* __restrict__ min_next, float band,
+                                                          const uint32_t* __restrict__ prev_taken, uint32_t release,
+                                                          int sign) {
     __shared__ uint32_t wave_tot[4], wave_min[4];
     __shared__ uint32_t block_base;
     const uint32_t ncells = g.n[0] * g.n[1] * g.n[2];  // below 2^32 (checked on the host)
@@ -259,7 +268,8 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
     if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
     const float mf = bitsf(spread_min(min_cur));
     float limit = mf + band;
-    if (!(limit >= mf)) limit = INFINITY;  // a NaN band (or no flagged cell) takes every flagged cell
+    // a NaN band (or no flagged cell) or a thin previous front: every flagged cell
+    if (!(limit >= mf) || *prev_taken < release) limit = INFINITY;
     const uint32_t lbits = fbits(limit);  // non-negative floats order as their bits
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t tile = blockIdx.x * (uint32_t)COMPACT_TILE; tile < ncells; tile += gridDim.x * (uint32_t)COMPACT_TILE) {
@@ -338,13 +348,80 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
     }
 }
 
-// Tries each item's triangle on the item's 26 neighbours (one lane per pair);
-// a strictly smaller key is stored with atomicMin and flags the cell for a later
-// round. The read before the atomic is a plain (cacheable) load: keys only
-// decrease, so a stale value can only cause an unneeded atomic. The smallest
-// improved distance is folded into min_next; min_cur (read by this round's
-// compaction, written by the next one) is reset to "none". The frontier size
-// goes to *total_out for the host.
+// Per-wave queue of (neighbour, triangle) pairs that passed the cheap tests: a
+// wave fills it 64 candidates at a time and evaluates 64 queued pairs whenever
+// it holds that many, so the distance evaluations run with every lane busy
+// instead of idling the lanes whose candidate was skipped.
+constexpr uint32_t PROP_QUEUE = 128;  // a fill adds at most 64 to at most 63 left over
+constexpr uint32_t PROP_WAVES = 4;    // 256-thread blocks
+
+struct PropQueue {
+    uint32_t x[PROP_WAVES][PROP_QUEUE], y[PROP_WAVES][PROP_QUEUE], z[PROP_WAVES][PROP_QUEUE];
+    uint32_t tri[PROP_WAVES][PROP_QUEUE], cur_lo[PROP_WAVES][PROP_QUEUE], cur_hi[PROP_WAVES][PROP_QUEUE];
+};
+
+// LDS traffic between lanes of one wave: keep the compiler from moving the
+// accesses across this point (the wave's LDS operations execute in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct PropAcc {
+    uint32_t n_evals = 0, imin = UINT32_MAX;
+    bool nan_seen = false;
+};
+
+// Evaluates one queued pair: the reference's distance of cell (x, y, z) to the
+// triangle, kept if its key is below the neighbour's current key.
+template <int SIGN>
+__device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const GridDev& g, uint64_t* keys,
+                                          uint8_t* __restrict__ changed, uint8_t* __restrict__ tile_changed,
+                                          uint32_t x, uint32_t y, uint32_t z, uint32_t tri, uint64_t cur,
+                                          PropAcc& acc) {
+    const TriRec tr = tris[tri];
+    const float px = cell_coord(g, 0, x), py = cell_coord(g, 1, y), pz = cell_coord(g, 2, z);
+    float dx, dy, dz, cx, cy, cz;
+    closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
+    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+    acc.n_evals++;
+    if (cur != KEY_EMPTY) {
+        // The key holds |d| = c. sqrt is monotone and correctly rounded, so
+        // d2 >= (next float above c)^2 gives |d| > c: no improvement, and the
+        // square root and sign can be skipped. (c is kept in the range where
+        // these squares are normal floats.)
+        const uint32_t cb = (uint32_t)(cur >> 32);
+        if (cb >= 0x26800000u && cb < 0x5f000000u) {  // 2^-50 <= c < 2^63
+            const float cp = bitsf(cb + 1u);
+            const float lim = bitsf(fbits(cp * cp) + 1u);  // >= the exact square of cp
+            if (d2 >= lim) return;
+        }
+    }
+    const float dist = sqrtf(d2);
+    float d = dist;
+    if (SIGN == SDFX_SIGN_NORMAL) d = dot3(dx, dy, dz, tr.nx, tr.ny, tr.nz) > 0.0f ? dist : -dist;
+    acc.nan_seen |= d != d;
+    if (!key_valid(d)) return;
+    const uint64_t key = make_key<SIGN>(d, tri);
+    if (key >= cur) return;
+    const uint32_t ci = (x * g.n[1] + y) * g.n[2] + z;
+    const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
+    if (key < old) {
+        changed[ci] = 1;
+        tile_changed[ci / COMPACT_TILE] = 1;
+        acc.imin = min(acc.imin, (uint32_t)(key >> 32));
+    }
+}
+
+// Tries each item's triangle on the item's 26 neighbours: one lane per pair
+// tests bounds and the neighbour's current key, the pairs left go through the
+// wave's queue to prop_eval. A strictly smaller key is stored with atomicMin
+// and flags the cell for a later round. The reads before the atomic are plain
+// (cacheable) loads: keys only decrease, so a stale value can only cost an
+// unneeded evaluation or atomic. The smallest improved distance is folded into
+// min_next; min_cur (read by this round's compaction, written by the next one)
+// is reset to "none". The frontier size goes to *total_out for the host.
 template <int SIGN>
 __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
                                                             const uint32_t* __restrict__ ctr_cur,
@@ -357,6 +434,7 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
                                                             unsigned long long* evals) {
     // prefix[c]: items in the classes before c
     __shared__ uint32_t prefix[NCLASS + 1];
+    __shared__ PropQueue Q;
     if (threadIdx.x < 64u) {
         const uint32_t c = ctr_cur[threadIdx.x * WORD_STRIDE];
         uint32_t x = c;
@@ -380,41 +458,63 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
     const uint32_t chunk = (work + XCDS - 1) / XCDS;
     const uint32_t j0 = (blockIdx.x % XCDS) * chunk, j1 = min(work, j0 + chunk);
     const uint32_t stride = gridDim.x / XCDS * blockDim.x;
-    bool nan_seen = false;
-    uint32_t n_evals = 0, imin = UINT32_MAX;
-    for (uint32_t j = j0 + blockIdx.x / XCDS * blockDim.x + threadIdx.x; j < j1; j += stride) {
-        const uint32_t f = j / 26u;
-        uint32_t k = j - f * 26u;
-        k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
-        uint32_t cls = 0;  // the last class whose prefix is <= f (a non-empty one)
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    PropAcc acc;
+    uint32_t qn = 0;  // queued pairs (the same value in every lane of the wave)
+    for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
+        const uint32_t j = base + lane;
+        bool keep = false;
+        uint32_t x = 0, y = 0, z = 0, tri = 0;
+        uint64_t cur = 0;
+        if (j < j1) {
+            const uint32_t f = j / 26u;
+            uint32_t k = j - f * 26u;
+            k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
+            uint32_t cls = 0;  // the last class whose prefix is <= f (a non-empty one)
 #pragma unroll
-        for (uint32_t step = NCLASS / 2; step; step >>= 1)
-            if (prefix[cls + step] <= f) cls += step;
-        const FrontierItem it = items[cls * region + (f - prefix[cls])];
-        const uint32_t x = it.x + k / 9u - 1u, y = it.y + (k / 3u) % 3u - 1u, z = it.z + k % 3u - 1u;
-        if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;  // -1 wraps to 2^32 - 1
-        const uint32_t ci = (x * g.n[1] + y) * g.n[2] + z;  // grid.rs:276-278
-        const uint64_t cur = keys[ci];
-        // A neighbour already holding this triangle holds exactly the key the
-        // triangle would give it, and keys only decrease: nothing to try.
-        if (cur != KEY_EMPTY && key_tri(cur, SIGN) == it.tri) continue;
-        const TriRec tr = tris[it.tri];
-        float d = cell_dist<SIGN>(g, x, y, z, tr);
-        n_evals++;
-        nan_seen |= d != d;
-        if (!key_valid(d)) continue;
-        const uint64_t key = make_key<SIGN>(d, it.tri);
-        if (key >= cur) continue;
-        const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
-        if (key < old) {
-            changed[ci] = 1;
-            tile_changed[ci / COMPACT_TILE] = 1;
-            imin = min(imin, (uint32_t)(key >> 32));
+            for (uint32_t step = NCLASS / 2; step; step >>= 1)
+                if (prefix[cls + step] <= f) cls += step;
+            const FrontierItem it = items[cls * region + (f - prefix[cls])];
+            x = it.x + k / 9u - 1u;
+            y = it.y + (k / 3u) % 3u - 1u;
+            z = it.z + k % 3u - 1u;
+            tri = it.tri;
+            if (x < g.n[0] && y < g.n[1] && z < g.n[2]) {  // -1 wraps to 2^32 - 1
+                cur = keys[(x * g.n[1] + y) * g.n[2] + z];  // grid.rs:276-278
+                // A neighbour already holding this triangle holds exactly the
+                // key the triangle would give it, and keys only decrease.
+                keep = cur == KEY_EMPTY || key_tri(cur, SIGN) != tri;
+            }
+        }
+        const uint64_t m = __ballot(keep);
+        if (keep) {
+            const uint32_t pos = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            Q.x[wave][pos] = x;
+            Q.y[wave][pos] = y;
+            Q.z[wave][pos] = z;
+            Q.tri[wave][pos] = tri;
+            Q.cur_lo[wave][pos] = (uint32_t)cur;
+            Q.cur_hi[wave][pos] = (uint32_t)(cur >> 32);
+        }
+        qn += (uint32_t)__popcll(m);
+        wave_lds_sync();
+        if (qn >= 64u) {  // evaluate the newest 64 queued pairs
+            qn -= 64u;
+            const uint32_t e = qn + lane;
+            const uint64_t c = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
+            prop_eval<SIGN>(tris, g, keys, changed, tile_changed, Q.x[wave][e], Q.y[wave][e], Q.z[wave][e],
+                            Q.tri[wave][e], c, acc);
+            wave_lds_sync();
         }
     }
-    if (nan_seen) atomicOr(&flags->nan_distance, 1u);
-    block_min_u32(min_next + (blockIdx.x % NCLASS) * WORD_STRIDE, imin);
-    if (evals) block_count(evals, n_evals);
+    if (lane < qn) {
+        const uint64_t c = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
+        prop_eval<SIGN>(tris, g, keys, changed, tile_changed, Q.x[wave][lane], Q.y[wave][lane], Q.z[wave][lane],
+                        Q.tri[wave][lane], c, acc);
+    }
+    if (acc.nan_seen) atomicOr(&flags->nan_distance, 1u);
+    block_min_u32(min_next + (blockIdx.x % NCLASS) * WORD_STRIDE, acc.imin);
+    if (evals) block_count(evals, acc.n_evals);
 }
 
 // ---------------------------------------------------------------- raycast sign
@@ -611,28 +711,32 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     FrontierItem* items = A.alloc<FrontierItem>((uint64_t)region * NCLASS);
     fill_u32(ctr[0], 2 * spread, 0u, s);
     fill_u32(minw[0], 2 * spread, UINT32_MAX, s);
-    fill_u32(totals, 2, 0u, s);
+    fill_u32(totals, 2, UINT32_MAX, s);  // round 0 reads totals[1]: "no thin front before it"
     SDFX_LAUNCH(k_mark_seeded, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, changed, tile_changed,
                 (uint64_t)COMPACT_TILE, minw[0]);
     SDFX_HIP(hipGetLastError());
     const float band =
         GRID_BAND_CELLS * std::fmax(std::fmax(std::fabs(g.size[0]), std::fabs(g.size[1])), std::fabs(g.size[2]));
+    const uint32_t release = (uint32_t)std::max<uint64_t>(GRID_RELEASE_MIN, ncells / GRID_RELEASE_DIV);
     const unsigned prop_blocks = 4096;
     const unsigned compact_blocks = (unsigned)std::min<uint64_t>(ntiles, 4096);
-    const int batch = 8;
+    int batch = 8;
     // Every round takes at least the nearest flagged cell and every improvement
     // strictly lowers a key, so the loop ends; the cap only turns an
     // inconsistency into an error instead of a hang.
     const uint64_t max_rounds = 256ull * (g.n[0] + g.n[1] + g.n[2]) + 4096;
     // Batches are enqueued one ahead of the one being checked: while the host
-    // waits for batch k's frontier size, batch k+1 is already on the GPU.
+    // waits for batch k's frontier size, batch k+1 is already on the GPU. Once
+    // the front thins out the batches shrink, so fewer empty rounds run after
+    // the last one that took a cell.
     BatchProbe probe;
     int cur = 0;
     uint64_t round = 0;
     auto enqueue_batch = [&](int slot) {
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact, dim3(compact_blocks), dim3(256), 0, s, keys, g, changed, tile_changed,
-                        items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1], band, (int)SIGN);
+                        items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1], band, totals + (cur ^ 1),
+                        release, (int)SIGN);
             SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
                         d_tris, g, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], totals + cur, d_flags,
                         evals ? evals + seed_slots : nullptr);
@@ -645,7 +749,9 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     for (int slot = 0;; slot ^= 1) {
         if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
         enqueue_batch(slot ^ 1);
-        if (probe.wait(slot) == 0) break;  // the batch after it only runs empty rounds
+        const uint32_t taken = probe.wait(slot);
+        if (taken == 0) break;  // the batch after it only runs empty rounds
+        batch = taken < 512 ? 2 : (taken < 8192 ? 4 : 8);
     }
 
     // 3. sign

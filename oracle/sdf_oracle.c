@@ -1061,6 +1061,9 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
         for (uint64_t i = 0; i < ncells; ++i) seed_out[i] = key_value(keys[i], sign);
     for (uint64_t i = 0; i < ncells; ++i) changed[i] = keys[i] != KEY_EMPTY;
     const float band = ORC_GRID_BAND_CELLS * fmaxf(fmaxf(fabsf(g->size.x), fabsf(g->size.y)), fabsf(g->size.z));
+    const uint64_t release = ncells / ORC_GRID_RELEASE_DIV > ORC_GRID_RELEASE_MIN ? ncells / ORC_GRID_RELEASE_DIV
+                                                                                  : ORC_GRID_RELEASE_MIN;
+    uint64_t prev_taken = UINT64_MAX; /* round 0 keeps the band */
     int64_t rounds = 0;
     uint64_t n_evals = 0;
     for (;;) {
@@ -1072,7 +1075,7 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
         float mf;
         memcpy(&mf, &m, 4);
         float limit = mf + band;
-        if (!(limit >= mf)) limit = INFINITY; /* a NaN band takes every flagged cell */
+        if (!(limit >= mf) || prev_taken < release) limit = INFINITY; /* a NaN band takes every flagged cell */
         uint64_t nf = 0;
         for (uint64_t i = 0; i < ncells; ++i) {
             if (!changed[i]) continue;
@@ -1086,6 +1089,7 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
             nf++;
         }
         rounds++;
+        prev_taken = nf;
 #pragma omp parallel for num_threads(n_threads) schedule(dynamic, 256) reduction(| : nan_seen) reduction(+ : n_evals)
         for (int64_t f = 0; f < (int64_t)nf; ++f) {
             uint64_t ci = fr_cell[f];
