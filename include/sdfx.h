@@ -42,6 +42,8 @@ typedef enum sdfx_status {
     /* Not an error: the reference returns an empty Vec / None here
      * (generate/generic/rtree_bvh.rs:96-98, 124-126). *out_count is 0. */
     SDFX_EMPTY_RESULT = 1,
+    /* Not an error: check_mesh_triangle_list returned Err (the message says why). */
+    SDFX_MESH_INVALID = 2,
     /* lib.rs:375 "Query point {..} contains non-finite values" */
     SDFX_ERR_NON_FINITE_QUERY = -1,
     /* lib.rs:203 "TriangleList indices length ({}) must be divisible by 3" */
@@ -142,6 +144,18 @@ sdfx_status sdfx_query_vector_to_closest_point(const sdfx_mesh* mesh, const floa
 
 /* Drops an SdfAccelerationMesh (lib.rs:303-312 is Clone + Drop). NULL is allowed. */
 void sdfx_free_sdf_acceleration(sdfx_mesh* mesh);
+
+/* Replaces generate/mesh_check.rs:24-81 check_mesh_triangle_list(vertices,
+ * faces) -> Result<(), String> (re-exported at lib.rs:149) for u32 faces.
+ * Returns SDFX_OK for Ok(()) and SDFX_MESH_INVALID for Err, with the reference's
+ * message in `message` (NUL-terminated, truncated to message_capacity). The
+ * optional kind_out gets 0 (Ok), 1 (face count not a multiple of 3),
+ * 2 (duplicate vertex), 3 (duplicate face), 4 (duplicate edge) or 5 (edge
+ * without its reverse); where_out the vertex index, face index or edge index
+ * 3 * face + k of edge (face[k], face[(k + 1) % 3]). Runs on the device. */
+sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex_count, const uint32_t* faces,
+                                          uint64_t face_index_count, int* kind_out, uint64_t* where_out,
+                                          char* message, uint64_t message_capacity);
 
 /* ---- Tuning (no counterpart in the reference). Process-wide knobs; returns 0
  * on success, -1 for an unknown key.

@@ -8,6 +8,8 @@ grid.rs, generate/grid.rs:265) over the C ABI of libsdfx.so:
     generate_grid_sdf(vertices, topology, Grid.from_bounding_box(lo, hi, n),
                       SignMethod.Raycast)
     build_sdf_acceleration / query_sdf / query_vector_to_closest_point_rtree_bvh
+    check_mesh_triangle_list(vertices, faces)
+    serde.save_to_file / serde.read_from_file (mesh_to_sdf_amd.serde)
 
 Vertices and query points are (N, 3) float32 arrays, indices u16/u32 arrays.
 Where the reference panics, an SdfxError carrying the status is raised.
@@ -28,7 +30,7 @@ from ._lib import SdfxError
 __all__ = [
     "SignMethod", "AccelerationMethod", "ALL_SDF_ACCELERATION_METHODS", "Topology", "Grid", "SnapResult",
     "generate_sdf", "generate_grid_sdf", "build_sdf_acceleration", "query_sdf",
-    "query_vector_to_closest_point_rtree_bvh", "SdfAccelerationMesh", "SdfxError",
+    "query_vector_to_closest_point_rtree_bvh", "SdfAccelerationMesh", "SdfxError", "check_mesh_triangle_list",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -306,3 +308,17 @@ def query_vector_to_closest_point_rtree_bvh(acceleration: SdfAccelerationMesh, q
     _lib.check(_lib.lib().sdfx_query_vector_to_closest_point(acceleration._h, _ptr(q, _F), len(q), _ptr(vec, _F),
                                                              _ptr(inside, _U8)))
     return vec, inside.astype(bool)
+
+
+def check_mesh_triangle_list(vertices, faces) -> Optional[str]:
+    """generate/mesh_check.rs:24-81 (re-exported at lib.rs:149), run on the
+    device: None for the reference's Ok(()), the reference's Err message
+    otherwise (duplicate vertex / face, duplicate or unmatched edge)."""
+    v = _verts(vertices)
+    f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1)
+    kind = ctypes.c_int(0)
+    where = ctypes.c_uint64(0)
+    msg = ctypes.create_string_buffer(512)
+    st = _lib.check(_lib.lib().sdfx_check_mesh_triangle_list(_ptr(v, _F), len(v), _ptr(f, _U32), f.size,
+                                                             ctypes.byref(kind), ctypes.byref(where), msg, 512))
+    return None if st == _lib.OK else msg.value.decode()

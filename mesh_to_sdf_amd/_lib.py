@@ -15,6 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 
 OK = 0
 EMPTY_RESULT = 1
+MESH_INVALID = 2
 ERR_NON_FINITE_QUERY = -1
 ERR_INDEX_COUNT = -2
 ERR_VERTEX_COUNT = -3
@@ -46,6 +47,8 @@ SIGNATURES = {
     "sdfx_query_sdf": (_i, [_vp, _f, _u64v, _f, _u64]),
     "sdfx_query_vector_to_closest_point": (_i, [_vp, _f, _u64v, _f, _u8]),
     "sdfx_free_sdf_acceleration": (None, [_vp]),
+    "sdfx_check_mesh_triangle_list": (_i, [_f, _u64v, _u32, _u64v, ctypes.POINTER(_i), _u64, ctypes.c_char_p,
+                                          _u64v]),
     "sdfx_profile_enable": (None, [_i]),
     "sdfx_profile_read": (_i, [_i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _u64]),
     "sdfx_profile_counter": (ctypes.c_uint64, [ctypes.c_char_p]),

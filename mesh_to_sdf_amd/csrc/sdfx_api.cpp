@@ -4,6 +4,7 @@
 // (index out of range), then the path-specific cases.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -52,15 +53,6 @@ sdfx_status guarded(F&& f) {
 
 [[noreturn]] void fail(sdfx_status s, const std::string& msg) { throw Error(s, msg); }
 
-std::string fmt_f32(float x) {
-    if (std::isnan(x)) return "NaN";
-    if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
-    char buf[64];
-    std::snprintf(buf, sizeof buf, "%.9g", (double)x);
-    std::string s(buf);
-    if (s.find_first_of(".e") == std::string::npos) s += ".0";
-    return s;
-}
 
 std::string topology_error_message(sdfx_status st, uint64_t nv, uint64_t ni) {
     std::ostringstream o;
@@ -83,8 +75,8 @@ void host_check_finite(const float* q, uint64_t nq) {
     for (uint64_t i = 0; i < nq; ++i) {
         const float* p = q + 3 * i;
         if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]))
-            fail(SDFX_ERR_NON_FINITE_QUERY, "Query point [" + fmt_f32(p[0]) + ", " + fmt_f32(p[1]) + ", " +
-                                                fmt_f32(p[2]) + "] contains non-finite values");
+            fail(SDFX_ERR_NON_FINITE_QUERY, "Query point [" + rust_debug_f32(p[0]) + ", " + rust_debug_f32(p[1]) + ", " +
+                                                rust_debug_f32(p[2]) + "] contains non-finite values");
     }
 }
 
@@ -434,6 +426,25 @@ void sdfx_free_sdf_acceleration(sdfx_mesh* mesh) {
     if (!mesh) return;
     (void)hipSetDevice(mesh->device);
     delete mesh;
+}
+
+sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex_count, const uint32_t* faces,
+                                          uint64_t face_index_count, int* kind_out, uint64_t* where_out,
+                                          char* message, uint64_t message_capacity) {
+    return guarded([&]() -> sdfx_status {
+        require(vertices || vertex_count == 0, "vertices");
+        require(faces || face_index_count == 0, "faces");
+        const MeshCheck r = check_mesh_triangle_list(vertices, vertex_count, faces, face_index_count,
+                                                     thread_stream());
+        if (kind_out) *kind_out = r.kind;
+        if (where_out) *where_out = r.where;
+        if (message && message_capacity) {
+            const uint64_t n = std::min<uint64_t>(r.message.size(), message_capacity - 1);
+            std::memcpy(message, r.message.data(), n);
+            message[n] = '\0';
+        }
+        return r.kind ? SDFX_MESH_INVALID : SDFX_OK;
+    });
 }
 
 }  // extern "C"

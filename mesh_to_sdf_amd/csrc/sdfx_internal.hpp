@@ -1,6 +1,8 @@
 // sdfx_internal.hpp — interfaces between libsdfx's translation units.
 #pragma once
 
+#include <string>
+
 #include "sdfx_geo.hpp"
 #include "sdfx_runtime.hpp"
 
@@ -75,6 +77,24 @@ void lbvh_closest_vectors(const Lbvh& bvh, const float* d_queries, uint64_t n_qu
 // queries x triangles is small enough that no acceleration structure pays.
 void exhaustive_query(const TriRec* d_tris, uint64_t n_tris, const float* d_queries, uint64_t n_queries,
                       QueryMode mode, float* d_out, DevFlags* d_flags, Arena& arena);
+
+// ---------------------------------------------------------------- mesh check
+
+// generate/mesh_check.rs:24-81 on host arrays: kind 0 (Ok), 1 (face count),
+// 2 (duplicate vertex), 3 (duplicate face), 4 (duplicate edge), 5 (edge without
+// reverse); `where` is the vertex, face or edge (3 * face + k) index; message
+// is the reference's Err string.
+struct MeshCheck {
+    int kind = 0;
+    uint64_t where = 0;
+    std::string message;
+};
+MeshCheck check_mesh_triangle_list(const float* vertices, uint64_t n_vertices, const uint32_t* faces,
+                                   uint64_t n_face_indices, hipStream_t stream);
+
+// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; the
+// result ends in keys / vals.
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits, Arena& A);
 
 // ---------------------------------------------------------------- tuning
 

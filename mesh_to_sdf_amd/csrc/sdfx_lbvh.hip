@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint32_t* __res
 }
 
 // Sorts (keys, vals) by the low `bits` bits of keys; stable. Result ends in keys/vals.
-static void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits, Arena& A) {
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits, Arena& A) {
     if (n <= 1) return;
     uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
     uint32_t* k2 = A.alloc<uint32_t>(n);

@@ -2,6 +2,9 @@
 #include "sdfx_runtime.hpp"
 
 #include <cstring>
+#include <charconv>
+#include <cmath>
+#include <string>
 
 #include <cstdio>
 #include <map>
@@ -10,6 +13,36 @@
 #include <vector>
 
 namespace sdfx {
+
+std::string rust_debug_f32(float x) {
+    if (std::isnan(x)) return "NaN";
+    if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+    const bool neg = std::signbit(x);
+    const float a = std::fabs(x);
+    if (a == 0.0f) return neg ? "-0.0" : "0.0";
+    // shortest round-trip digits d1 d2 ... dk and decimal exponent e: a = d1.d2...dk * 10^e
+    char buf[48];
+    const auto r = std::to_chars(buf, buf + sizeof buf, a, std::chars_format::scientific);
+    const std::string sci(buf, r.ptr);
+    const size_t epos = sci.find('e');
+    std::string digits;
+    for (size_t i = 0; i < epos; ++i)
+        if (sci[i] != '.') digits += sci[i];
+    const int e = std::stoi(sci.substr(epos + 1));
+    std::string out = neg ? "-" : "";
+    if (a < 1e-4f || a >= 1e16f) {  // float_to_exponential_common_shortest
+        out += digits[0];
+        if (digits.size() > 1) out += "." + digits.substr(1);
+        return out + "e" + std::to_string(e);
+    }
+    if (e >= 0) {  // float_to_decimal_common_shortest with at least one fractional digit
+        std::string ip = digits.substr(0, std::min<size_t>(digits.size(), (size_t)e + 1));
+        ip.append((size_t)e + 1 > digits.size() ? (size_t)e + 1 - digits.size() : 0, '0');
+        std::string fp = (size_t)e + 1 < digits.size() ? digits.substr((size_t)e + 1) : "0";
+        return out + ip + "." + fp;
+    }
+    return out + "0." + std::string((size_t)(-e - 1), '0') + digits;
+}
 
 static thread_local std::string g_last_error;
 

@@ -21,6 +21,11 @@ struct Error : std::runtime_error {
     Error(sdfx_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
 };
 
+// Rust's `{:?}` of an f32 (core::fmt::float, float_to_general_debug): the
+// shortest digits that round-trip, "1.0"-style decimals for 1e-4 <= |x| < 1e16
+// and zero, "1.5e-7"-style exponentials otherwise, "NaN", "inf", "-inf", "-0.0".
+std::string rust_debug_f32(float x);
+
 void set_last_error(const std::string& msg);
 void clear_last_error();
 const std::string& last_error();

@@ -94,6 +94,8 @@ def lib() -> ctypes.CDLL:
             "orc_fp64_distance": (None, [_f, _u32, ctypes.c_uint64, _f, ctypes.c_uint64, _d, ctypes.c_int]),
             "orc_ray_parity_bruteforce": (None, [_f, _u32, ctypes.c_uint64, _f, ctypes.c_uint64, _u8, ctypes.c_int]),
             "orc_check_mesh_triangle_list": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64]),
+            "orc_check_mesh_triangle_list_at": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64,
+                                                                ctypes.POINTER(ctypes.c_uint64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -318,3 +320,13 @@ def check_mesh_triangle_list(vertices, faces):
     v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
     f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1)
     return lib().orc_check_mesh_triangle_list(_fp(v), len(v), f.ctypes.data_as(_u32), f.size)
+
+
+def check_mesh_triangle_list_at(vertices, faces):
+    """(kind, where): kind 0..5 as check_mesh_triangle_list; where is the failing
+    vertex index (2), face index (3) or edge index 3 * face + k (4, 5)."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1)
+    w = ctypes.c_uint64(0)
+    k = lib().orc_check_mesh_triangle_list_at(_fp(v), len(v), f.ctypes.data_as(_u32), f.size, ctypes.byref(w))
+    return k, int(w.value)

@@ -1456,25 +1456,41 @@ static int vertex_key_eq(const float* a, const float* b) {
     return 1;
 }
 
-/* generate/mesh_check.rs:24-81. Returns 0 (Ok), 1 (not a multiple of 3),
- * 2 (duplicate vertex), 3 (duplicate face), 4 (duplicate edge), 5 (open edge). */
-int orc_check_mesh_triangle_list(const float* verts, uint64_t nv, const uint32_t* faces, uint64_t nf) {
+/* generate/mesh_check.rs:24-81, as written: O(V^2) duplicate vertices (the
+ * first vertex equal to an earlier one, :37-43), O(F^2) duplicate faces (the
+ * first face equal to an earlier one as an ordered triple, :46-49), then the
+ * edge loop of :52-78 that pops the last edge, fails on a remaining copy of it
+ * or on a missing reverse edge, and removes every copy of the reverse.
+ * Returns 0 (Ok), 1 (not a multiple of 3), 2 (duplicate vertex), 3 (duplicate
+ * face), 4 (duplicate edge), 5 (edge without its reverse); *where_out gets the
+ * vertex index (2), the face index (3) or the edge's index 3 * face + k for the
+ * edge (face[k], face[(k + 1) % 3]) (4, 5). */
+int orc_check_mesh_triangle_list_at(const float* verts, uint64_t nv, const uint32_t* faces, uint64_t nf,
+                                    uint64_t* where_out) {
+    *where_out = 0;
     if (nf % 3 != 0) return 1;
     for (uint64_t i = 0; i < nv; ++i)
         for (uint64_t j = 0; j < i; ++j)
-            if (vertex_key_eq(verts + 3 * i, verts + 3 * j)) return 2;
+            if (vertex_key_eq(verts + 3 * i, verts + 3 * j)) {
+                *where_out = i;
+                return 2;
+            }
     uint64_t nt = nf / 3;
     for (uint64_t i = 0; i < nt; ++i)
         for (uint64_t j = 0; j < i; ++j)
             if (faces[3 * i] == faces[3 * j] && faces[3 * i + 1] == faces[3 * j + 1] &&
-                faces[3 * i + 2] == faces[3 * j + 2])
+                faces[3 * i + 2] == faces[3 * j + 2]) {
+                *where_out = i;
                 return 3;
+            }
     uint64_t ne = 3 * nt;
     uint32_t* e = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (ne + 1));
+    uint64_t* origin = (uint64_t*)malloc(sizeof(uint64_t) * (ne + 1));
     for (uint64_t f = 0; f < nt; ++f)
         for (int k = 0; k < 3; ++k) {
             e[2 * (3 * f + k)] = faces[3 * f + k];
             e[2 * (3 * f + k) + 1] = faces[3 * f + (k + 1) % 3];
+            origin[3 * f + k] = 3 * f + (uint64_t)k;
         }
     int result = 0;
     while (ne) {
@@ -1488,10 +1504,12 @@ int orc_check_mesh_triangle_list(const float* verts, uint64_t nv, const uint32_t
         }
         if (same) {
             result = 4;
+            *where_out = origin[ne];
             break;
         }
         if (!has_opp) {
             result = 5;
+            *where_out = origin[ne];
             break;
         }
         uint64_t w = 0;
@@ -1499,10 +1517,17 @@ int orc_check_mesh_triangle_list(const float* verts, uint64_t nv, const uint32_t
             if (!(e[2 * i] == b && e[2 * i + 1] == a)) {
                 e[2 * w] = e[2 * i];
                 e[2 * w + 1] = e[2 * i + 1];
+                origin[w] = origin[i];
                 w++;
             }
         ne = w;
     }
     free(e);
+    free(origin);
     return result;
+}
+
+int orc_check_mesh_triangle_list(const float* verts, uint64_t nv, const uint32_t* faces, uint64_t nf) {
+    uint64_t where;
+    return orc_check_mesh_triangle_list_at(verts, nv, faces, nf, &where);
 }

@@ -134,6 +134,10 @@ void orc_ray_parity_bruteforce(const float* vertices, const uint32_t* tris, uint
 /* mesh_check.rs:24-81 (returns 0 when Ok, else an error kind 1..5) */
 int orc_check_mesh_triangle_list(const float* vertices, uint64_t n_vertices, const uint32_t* faces,
                                  uint64_t n_faces_idx);
+/* The same, with the failing vertex index (kind 2), face index (3) or edge
+ * index 3 * face + k (4, 5) in *where_out. */
+int orc_check_mesh_triangle_list_at(const float* vertices, uint64_t n_vertices, const uint32_t* faces,
+                                    uint64_t n_faces_idx, uint64_t* where_out);
 
 #ifdef __cplusplus
 }

@@ -79,3 +79,37 @@ PROPTEST_REGRESSIONS = [
     ([0.0, -5.8359632, 4.405388], [0.0, 0.9572999, 9.758267], [6.9999175, -4.739112, 7.5462694],
      [0.0, -9.673183, 0.52112055]),
 ]
+
+
+def rust_debug_f32(x) -> str:
+    """Rust's `{:?}` of an f32 (core::fmt::float: shortest round-trip digits;
+    exponential when the value is nonzero and below 1e-4 or at least 1e16),
+    built from numpy's shortest float32 digits: an independent statement of
+    the formatting the library does in C++."""
+    x = np.float32(x)
+    if np.isnan(x):
+        return "NaN"
+    if np.isinf(x):
+        return "inf" if x > 0 else "-inf"
+    sign = "-" if np.signbit(x) else ""
+    a = np.float32(abs(x))
+    if a == 0:
+        return sign + "0.0"
+    mant, exp = np.format_float_scientific(a, unique=True, trim="-").split("e")
+    digits, e = mant.replace(".", ""), int(exp)
+    if a < np.float32(1e-4) or a >= np.float32(1e16):
+        return sign + digits[0] + ("." + digits[1:] if len(digits) > 1 else "") + "e" + str(e)
+    if e >= 0:
+        ip = digits[: e + 1].ljust(e + 1, "0")
+        fp = digits[e + 1:] or "0"
+        return sign + ip + "." + fp
+    return sign + "0." + "0" * (-e - 1) + digits
+
+
+# Known outputs of Rust's f32 Debug formatting.
+RUST_DEBUG_F32 = [
+    (0.0, "0.0"), (-0.0, "-0.0"), (1.0, "1.0"), (-1.0, "-1.0"), (0.1, "0.1"), (0.5, "0.5"), (123456.7, "123456.7"),
+    (16777217.0, "16777216.0"), (1e-4, "0.0001"), (1e-5, "1e-5"), (1.5e-7, "1.5e-7"), (1e15, "1000000000000000.0"),
+    (1e16, "1e16"), (3.4028235e38, "3.4028235e38"), (1e-45, "1e-45"), (float("nan"), "NaN"), (float("inf"), "inf"),
+    (float("-inf"), "-inf"), (2.5, "2.5"), (100.0, "100.0"), (0.001, "0.001"), (-0.00012345, "-0.00012345"),
+]

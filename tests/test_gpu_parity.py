@@ -180,3 +180,22 @@ def test_grid_partial_and_negative_size(sdf, oracle):
                                              oracle.GRID_FRONTIER)
         gpu = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), sdf.Grid(first, size, [20, 20, 20]), 0)
         assert_bit_exact(gpu, ref, f"grid size={size}")
+
+
+def test_grid_result_round_trips_through_serde(sdf, tmp_path):
+    """A device grid written in the reference's V1 format (serde.rs:77-221) reads back bit for bit."""
+    from mesh_to_sdf_amd import serde
+    v, f = S.icosphere(3)
+    first, size, count = S.grid_for(v, (24, 20, 16))
+    grid = sdf.Grid(first, size, count)
+    d = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), grid, sdf.SignMethod.Raycast)
+    path = tmp_path / "grid.bin"
+    serde.save_to_file(serde.SerializeSdf.Grid(grid, d), path)
+    de = serde.read_from_file(path)
+    assert de.grid == grid
+    assert np.array_equal(bits(de.distances), bits(d))
+    q = S.uniform_queries(v, 1000, 9)
+    dq = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, sdf.AccelerationMethod.RtreeBvh)
+    serde.save_to_file(serde.SerializeSdf.Generic(q, dq), path)
+    de = serde.read_from_file(path)
+    assert np.array_equal(bits(de.query_points), bits(q)) and np.array_equal(bits(de.distances), bits(dq))
