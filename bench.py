@@ -173,36 +173,33 @@ def main():
     dom_ms, dom_launches = _lib.profile_read()[dom_name]
     _lib.profile_enable(False)
     mesh_bytes = 12 * len(v) + 12 * len(f)
+    # Algorithmic bytes per SDF sample (implementation-independent): a grid cell
+    # writes 4 B, a point query reads 12 B and writes 4 B; the mesh (12 B per
+    # vertex and per index triple) is read once per call. One call runs
+    # `launches_per_step` launches of the dominant kernel, which are charged
+    # the call's bytes in equal parts.
+    per_sample = 4.0 if args.workload == "grid" else 16.0
+    step_bytes = per_sample * samples + mesh_bytes
+    launches_per_step = dom_launches / args.steps
+    avg_launch_s = dom_ms / dom_launches / 1e3
+    achieved = step_bytes / launches_per_step / avg_launch_s / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                "traffic": measured_traffic(dom_name), "kernel": dom_name, "avg_launch_ms": dom_ms / dom_launches,
+                "work_per_launch": f"{step_bytes / launches_per_step:.0f} B ({per_sample:.0f} B x {samples} samples "
+                                   f"+ {mesh_bytes} B of mesh, over {launches_per_step:g} launches per call)"}
+    valu = None
     if args.workload == "grid":
-        # 80 FP32 operations per point-triangle distance (geo.rs:26-30, 70-138 as written)
+        # diagnostic beside the headline: FP32 VALU work of the distance evaluations
+        # (80 operations per point-triangle distance, geo.rs:26-30, 70-138 as written)
         evals = None
         if dom_name.startswith("k_frontier_propagate"):
             evals = counters["grid.propagate_distance_evals"]
         elif dom_name.startswith("k_seed_pairs"):
             evals = counters["grid.seed_distance_evals"]
-        if evals is not None:
-            flops_per_launch = 80.0 * evals / dom_launches
-            achieved = flops_per_launch / (dom_ms / dom_launches / 1e3) / 1e12
-            roofline = {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
-                        "frac": achieved / 157.3, "traffic": measured_traffic(dom_name), "kernel": dom_name,
-                        "engine": "FP32 VALU (no MFMA: 157.3 TFLOP/s is MI355X's FP32 vector peak)",
-                        "work_per_launch": f"{evals / dom_launches:.0f} distance evaluations x 80 FLOP",
-                        "avg_launch_ms": dom_ms / dom_launches}
-        else:
-            achieved = (4.0 * samples + mesh_bytes) / (dom_ms / dom_launches / 1e3) / 1e9
-            roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                        "frac": achieved / 8000.0, "traffic": measured_traffic(dom_name), "kernel": dom_name}
-        step_bytes = 4.0 * samples + mesh_bytes
-        roofline_hbm_step = {"bytes_per_step": step_bytes, "GB/s": step_bytes / (ms / 1e3) / 1e9,
-                             "frac": step_bytes / (ms / 1e3) / 1e9 / 8000.0,
-                             "per_sample_bytes": step_bytes / samples}
-    else:
-        per_launch = 16.0 * samples + mesh_bytes  # 12 B query + 4 B result per sample, mesh once
-        achieved = per_launch / (dom_ms / dom_launches / 1e3) / 1e9
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                    "frac": achieved / 8000.0, "traffic": measured_traffic(dom_name), "kernel": dom_name,
-                    "avg_launch_ms": dom_ms / dom_launches}
-        roofline_hbm_step = None
+        if evals:
+            tflops = 80.0 * evals / dom_launches / avg_launch_s / 1e12
+            valu = {"achieved": tflops, "peak": 157.3, "unit": "TFLOP/s", "frac": tflops / 157.3,
+                    "evals_per_launch": evals / dom_launches}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, v, f, locals())
 
@@ -224,8 +221,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {n: round(t / args.steps, 4) for n, t, _ in kernels[:8]},
     }
-    if roofline_hbm_step:
-        result["roofline_hbm_step"] = roofline_hbm_step
+    if valu:
+        result["valu_distance_evals"] = valu
     if counters["lbvh.queries"]:
         nq_total = counters["lbvh.queries"]
         result["traversal_per_query"] = {k.split(".", 1)[1]: counters[k] / nq_total for k in (

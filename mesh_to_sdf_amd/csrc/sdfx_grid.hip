@@ -421,7 +421,8 @@ __device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const
 // (cacheable) loads: keys only decrease, so a stale value can only cost an
 // unneeded evaluation or atomic. The smallest improved distance is folded into
 // min_next; min_cur (read by this round's compaction, written by the next one)
-// is reset to "none". The frontier size goes to *total_out for the host.
+// is reset to "none". The frontier size goes to *total_out for the next round's
+// compaction and, when host_total is not null, to that pinned host word.
 template <int SIGN>
 __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
                                                             const uint32_t* __restrict__ ctr_cur,
@@ -430,7 +431,8 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
                                                             uint8_t* __restrict__ tile_changed,
                                                             uint32_t* __restrict__ min_next,
                                                             uint32_t* __restrict__ min_cur,
-                                                            uint32_t* __restrict__ total_out, DevFlags* flags,
+                                                            uint32_t* __restrict__ total_out,
+                                                            uint32_t* __restrict__ host_total, DevFlags* flags,
                                                             unsigned long long* evals) {
     // prefix[c]: items in the classes before c
     __shared__ uint32_t prefix[NCLASS + 1];
@@ -447,7 +449,13 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
         if (threadIdx.x == 0) prefix[0] = 0;
         if (blockIdx.x == 0) {
             min_cur[threadIdx.x * WORD_STRIDE] = UINT32_MAX;
-            if (threadIdx.x == 63) *total_out = x;
+            if (threadIdx.x == 63) {
+                *total_out = x;
+                if (host_total) {
+                    *host_total = x;
+                    __threadfence_system();
+                }
+            }
         }
     }
     __syncthreads();
@@ -628,11 +636,14 @@ __global__ void __launch_bounds__(256) k_grid_finish(const uint64_t* __restrict_
 
 // ---------------------------------------------------------------- driver
 
-// Two pinned words per thread and their events: the host reads the frontier
-// size of one batch while the next batch runs.
+// Two pinned words per thread and their events: the last propagation of a
+// batch stores its frontier size straight into one of the words (no copy
+// kernel), and the host reads it after the batch's event while the next batch
+// runs.
 struct BatchProbe {
     struct Slots {
-        uint32_t* words = nullptr;
+        uint32_t* words = nullptr;    // host address
+        uint32_t* d_words = nullptr;  // the same words as the device addresses them
         hipEvent_t ev[2] = {nullptr, nullptr};
         ~Slots() {
             if (words) (void)hipHostFree(words);
@@ -643,16 +654,15 @@ struct BatchProbe {
     Slots& slots() {
         static thread_local Slots sl;
         if (!sl.words) {
-            SDFX_HIP(hipHostMalloc((void**)&sl.words, 2 * sizeof(uint32_t), hipHostMallocDefault));
+            SDFX_HIP(hipHostMalloc((void**)&sl.words, 2 * sizeof(uint32_t),
+                                   hipHostMallocMapped | hipHostMallocCoherent));
+            SDFX_HIP(hipHostGetDevicePointer((void**)&sl.d_words, sl.words, 0));
             for (auto& e : sl.ev) SDFX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
         return sl;
     }
-    void record(int slot, const uint32_t* d_count, hipStream_t s) {
-        Slots& sl = slots();
-        SDFX_HIP(hipMemcpyAsync(&sl.words[slot], d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        SDFX_HIP(hipEventRecord(sl.ev[slot], s));
-    }
+    uint32_t* word(int slot) { return &slots().d_words[slot]; }
+    void record(int slot, hipStream_t s) { SDFX_HIP(hipEventRecord(slots().ev[slot], s)); }
     uint32_t wait(int slot) {
         Slots& sl = slots();
         SDFX_HIP(hipEventSynchronize(sl.ev[slot]));
@@ -738,12 +748,12 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
                         items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1], band, totals + (cur ^ 1),
                         release, (int)SIGN);
             SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
-                        d_tris, g, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], totals + cur, d_flags,
-                        evals ? evals + seed_slots : nullptr);
+                        d_tris, g, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], totals + cur,
+                        b == batch - 1 ? probe.word(slot) : nullptr, d_flags, evals ? evals + seed_slots : nullptr);
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());
-        probe.record(slot, totals + (cur ^ 1), s);  // size of the batch's last frontier
+        probe.record(slot, s);  // after the batch's last frontier size is stored
     };
     enqueue_batch(0);
     for (int slot = 0;; slot ^= 1) {
