@@ -121,6 +121,34 @@ sdfx_status sdfx_generate_grid_sdf_device(const float* d_vertices, uint64_t vert
                                           const uint64_t cell_count[3], sdfx_sign_method sign, float* d_out,
                                           void* stream);
 
+/* One request of a batched generate_grid_sdf (the fields of
+ * sdfx_generate_grid_sdf's arguments); `status` is set by the call. */
+typedef struct sdfx_grid_request {
+    const float* vertices;
+    uint64_t vertex_count;
+    const uint32_t* indices; /* NULL: no index buffer */
+    uint64_t index_count;
+    sdfx_topology topology;
+    float first_cell[3];
+    float cell_size[3];
+    uint64_t cell_count[3];
+    float* out;
+    sdfx_status status;
+} sdfx_grid_request;
+
+/* Batched generate_grid_sdf (no counterpart in the reference, which takes one
+ * mesh per call; for callers with many meshes): every request, all with one
+ * sign method, runs through the same device launches, and each output is
+ * bit-identical to sdfx_generate_grid_sdf on that request alone. Returns
+ * SDFX_OK when every request succeeded, else the first failing request's status
+ * (each request's own status is in its `status`; sdfx_last_error names the
+ * request). Pointers are host pointers; the _device form takes device pointers
+ * and a stream. */
+sdfx_status sdfx_generate_grid_sdf_batch(sdfx_grid_request* requests, uint64_t request_count,
+                                         sdfx_sign_method sign);
+sdfx_status sdfx_generate_grid_sdf_batch_device(sdfx_grid_request* requests, uint64_t request_count,
+                                                sdfx_sign_method sign, void* stream);
+
 /* Replaces lib.rs:410-433 build_sdf_acceleration(vertices, indices,
  * acceleration_method) -> Option<SdfAccelerationMesh>. *mesh_out is NULL with
  * SDFX_EMPTY_RESULT where the reference returns None (RtreeBvh on an empty

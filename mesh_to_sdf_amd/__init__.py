@@ -31,6 +31,7 @@ __all__ = [
     "SignMethod", "AccelerationMethod", "ALL_SDF_ACCELERATION_METHODS", "Topology", "Grid", "SnapResult",
     "generate_sdf", "generate_grid_sdf", "build_sdf_acceleration", "query_sdf",
     "query_vector_to_closest_point_rtree_bvh", "SdfAccelerationMesh", "SdfxError", "check_mesh_triangle_list",
+    "generate_grid_sdf_batch",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -250,6 +251,38 @@ def generate_grid_sdf(vertices, indices: Topology, grid: Grid, sign_method: int 
         _ptr(v, _F), len(v), _ptr(idx, _U32), 0 if idx is None else idx.size, indices.kind, _ptr(first, _F),
         _ptr(size, _F), _ptr(cnt, _U64), sign_method, _ptr(out, _F)))
     return out
+
+
+def generate_grid_sdf_batch(requests, sign_method: int = SignMethod.Raycast, return_status: bool = False):
+    """Many generate_grid_sdf calls (generate/grid.rs:265-273) in one device run:
+    `requests` is a sequence of (vertices, topology, grid). Each result is
+    bit-identical to generate_grid_sdf on that request alone. Raises the first
+    failing request's error, or with return_status=True returns
+    (results, statuses) with None for failed requests."""
+    reqs = (_lib.GridRequest * max(1, len(requests)))()
+    keep = []
+    outs = []
+    for r, (vertices, indices, grid) in zip(reqs, requests):
+        v = _verts(vertices)
+        idx = None if indices.indices is None else np.ascontiguousarray(indices.indices, dtype=np.uint32).reshape(-1)
+        out = np.empty(grid.get_total_cell_count(), dtype=np.float32)
+        keep += [v, idx]
+        outs.append(out)
+        r.vertices = v.ctypes.data
+        r.vertex_count = len(v)
+        r.indices = None if idx is None else idx.ctypes.data
+        r.index_count = 0 if idx is None else idx.size
+        r.topology = indices.kind
+        r.first_cell[:] = [float(x) for x in grid.first_cell]
+        r.cell_size[:] = [float(x) for x in grid.cell_size]
+        r.cell_count[:] = list(grid.cell_count)
+        r.out = out.ctypes.data
+    st = _lib.lib().sdfx_generate_grid_sdf_batch(reqs, len(requests), int(sign_method))
+    if return_status:
+        statuses = [reqs[i].status for i in range(len(requests))]
+        return [o if s == _lib.OK else None for o, s in zip(outs, statuses)], statuses
+    _lib.check(st)
+    return outs
 
 
 class SdfAccelerationMesh:

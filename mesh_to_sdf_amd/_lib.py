@@ -35,6 +35,14 @@ _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _u64v = ctypes.c_uint64
 
+class GridRequest(ctypes.Structure):
+    """sdfx_grid_request (include/sdfx.h)."""
+    _fields_ = [("vertices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint64), ("indices", ctypes.c_void_p),
+                ("index_count", ctypes.c_uint64), ("topology", ctypes.c_int), ("first_cell", ctypes.c_float * 3),
+                ("cell_size", ctypes.c_float * 3), ("cell_count", ctypes.c_uint64 * 3), ("out", ctypes.c_void_p),
+                ("status", ctypes.c_int)]
+
+
 # name -> (restype, argtypes); must match include/sdfx.h
 SIGNATURES = {
     "sdfx_last_error": (ctypes.c_char_p, []),
@@ -47,6 +55,8 @@ SIGNATURES = {
     "sdfx_query_sdf": (_i, [_vp, _f, _u64v, _f, _u64]),
     "sdfx_query_vector_to_closest_point": (_i, [_vp, _f, _u64v, _f, _u8]),
     "sdfx_free_sdf_acceleration": (None, [_vp]),
+    "sdfx_generate_grid_sdf_batch": (_i, [ctypes.POINTER(GridRequest), _u64v, _i]),
+    "sdfx_generate_grid_sdf_batch_device": (_i, [ctypes.POINTER(GridRequest), _u64v, _i, _vp]),
     "sdfx_check_mesh_triangle_list": (_i, [_f, _u64v, _u32, _u64v, ctypes.POINTER(_i), _u64, ctypes.c_char_p,
                                           _u64v]),
     "sdfx_profile_enable": (None, [_i]),

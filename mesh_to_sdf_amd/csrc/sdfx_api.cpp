@@ -11,6 +11,7 @@
 #include <memory>
 #include <new>
 #include <sstream>
+#include <vector>
 
 #include "sdfx_internal.hpp"
 
@@ -210,6 +211,108 @@ sdfx_status grid_core(const float* d_v, uint64_t nv, const uint32_t* d_idx, uint
     if (zero_axis && nt > 0) fail(SDFX_ERR_INVALID_GRID, "assertion failed: min <= max");
     if (f.nan_distance) fail_nan();
     return SDFX_OK;
+}
+
+// One request of a batched grid call after the host-side checks grid_core
+// makes (enums, topology counts, size limits); a failed check sets `status`.
+struct BatchPrep {
+    sdfx_status status = SDFX_OK;
+    std::string message;
+    uint64_t nt = 0;
+    bool zero_axis = false;
+    GridParams g{};
+};
+
+BatchPrep prepare_grid_request(const sdfx_grid_request& r, sdfx_sign_method sign) {
+    BatchPrep p;
+    try {
+        check_enums(r.topology, SDFX_ACCEL_NONE, sign);
+        if (!r.vertices && r.vertex_count) fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: vertices");
+        if (!r.out && r.cell_count[0] * r.cell_count[1] * r.cell_count[2] != 0)
+            fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: out");
+        sdfx_status ts = topology_triangle_count(r.vertex_count, r.indices != nullptr, r.index_count, r.topology, &p.nt);
+        if (ts != SDFX_OK) fail(ts, topology_error_message(ts, r.vertex_count, r.index_count));
+        for (int k = 0; k < 3; ++k) {
+            p.g.first[k] = r.first_cell[k];
+            p.g.size[k] = r.cell_size[k];
+            p.g.count[k] = r.cell_count[k];
+        }
+        p.zero_axis = r.cell_count[0] == 0 || r.cell_count[1] == 0 || r.cell_count[2] == 0;
+        if (!p.zero_axis) {
+            uint64_t c = r.cell_count[0];
+            bool big = false;
+            for (int k = 1; k < 3; ++k) {
+                if (r.cell_count[k] > (UINT64_MAX / c)) big = true;
+                else c *= r.cell_count[k];
+            }
+            if (big || c >= (1ull << 32) || r.cell_count[0] >= (1ull << 31) || r.cell_count[1] >= (1ull << 31) ||
+                r.cell_count[2] >= (1ull << 31))
+                fail(SDFX_ERR_INVALID_ARGUMENT, "grids are limited to fewer than 2^32 cells");
+        }
+    } catch (const Error& e) {
+        p.status = e.status;
+        p.message = e.what();
+    }
+    return p;
+}
+
+// Runs the requests whose device pointers are in `reqs`; fills each status.
+sdfx_status grid_batch_core(sdfx_grid_request* reqs, uint64_t n, sdfx_sign_method sign, hipStream_t s) {
+    std::vector<BatchPrep> prep(n);
+    uint64_t nt_total = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        prep[i] = prepare_grid_request(reqs[i], sign);
+        if (prep[i].status == SDFX_OK) nt_total += prep[i].nt;
+    }
+    DevBuf<TriRec> tris(nt_total);
+    std::vector<GridBatchItem> items;
+    Arena A(s);
+    DevFlags* flags = A.alloc<DevFlags>(n ? n : 1);
+    fill_u32(reinterpret_cast<uint32_t*>(flags), (n ? n : 1) * (sizeof(DevFlags) / sizeof(uint32_t)), 0u, s);
+    uint64_t off = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (prep[i].status != SDFX_OK) continue;
+        const sdfx_grid_request& r = reqs[i];
+        gather_triangles(r.vertices, r.vertex_count, r.indices, r.topology, prep[i].nt, tris.get() + off, flags + i,
+                         s);
+        if (!(prep[i].zero_axis && prep[i].nt > 0)) items.push_back(GridBatchItem{off, prep[i].nt, prep[i].g, r.out,
+                                                                                  (uint32_t)i});
+        off += prep[i].nt;
+    }
+    grid_sdf_batch(tris.get(), items, sign, flags, A);
+    std::vector<DevFlags> hf(n ? n : 1);
+    SDFX_HIP(hipMemcpyAsync(hf.data(), flags, (n ? n : 1) * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
+    SDFX_HIP(hipStreamSynchronize(s));
+    sdfx_status first = SDFX_OK;
+    std::string first_msg;
+    for (uint64_t i = 0; i < n; ++i) {
+        sdfx_status st = prep[i].status;
+        std::string msg = prep[i].message;
+        if (st == SDFX_OK) {  // in grid_core's order
+            const DevFlags& f = hf[i];
+            if (f.index_out_of_range) {
+                st = SDFX_ERR_INDEX_OUT_OF_RANGE;
+                msg = "index out of bounds: the len is " + std::to_string(reqs[i].vertex_count) +
+                      " but a triangle index is >= " + std::to_string(reqs[i].vertex_count);
+            } else if (f.internal) {
+                st = SDFX_ERR_DEVICE;
+                msg = "device consistency check failed (traversal stack or work decode)";
+            } else if (prep[i].zero_axis && prep[i].nt > 0) {
+                st = SDFX_ERR_INVALID_GRID;
+                msg = "assertion failed: min <= max";
+            } else if (f.nan_distance) {
+                st = SDFX_ERR_NAN_DISTANCE;
+                msg = "NaN distance";
+            }
+        }
+        reqs[i].status = st;
+        if (st != SDFX_OK && first == SDFX_OK) {
+            first = st;
+            first_msg = "request " + std::to_string(i) + ": " + msg;
+        }
+    }
+    if (first != SDFX_OK) set_last_error(first_msg);
+    return first;
 }
 
 template <typename T>
@@ -444,6 +547,45 @@ sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex
             message[n] = '\0';
         }
         return r.kind ? SDFX_MESH_INVALID : SDFX_OK;
+    });
+}
+
+sdfx_status sdfx_generate_grid_sdf_batch_device(sdfx_grid_request* requests, uint64_t request_count,
+                                                sdfx_sign_method sign, void* stream) {
+    return guarded([&]() -> sdfx_status {
+        require(requests || request_count == 0, "requests");
+        return grid_batch_core(requests, request_count, sign, (hipStream_t)stream);
+    });
+}
+
+sdfx_status sdfx_generate_grid_sdf_batch(sdfx_grid_request* requests, uint64_t request_count,
+                                         sdfx_sign_method sign) {
+    return guarded([&]() -> sdfx_status {
+        require(requests || request_count == 0, "requests");
+        hipStream_t s = thread_stream();
+        // device copies of every request's inputs and outputs
+        std::vector<DevBuf<float>> dv(request_count), dout(request_count);
+        std::vector<DevBuf<uint32_t>> di(request_count);
+        std::vector<sdfx_grid_request> dreq(requests, requests + request_count);
+        for (uint64_t i = 0; i < request_count; ++i) {
+            const sdfx_grid_request& r = requests[i];
+            const uint64_t nc = r.cell_count[0] * r.cell_count[1] * r.cell_count[2];
+            if (r.vertices) upload(dv[i], r.vertices, 3 * r.vertex_count, s);
+            if (r.indices) upload(di[i], r.indices, r.index_count, s);
+            dout[i].reset(nc);
+            dreq[i].vertices = r.vertices ? dv[i].get() : nullptr;
+            dreq[i].indices = r.indices ? di[i].get() : nullptr;
+            dreq[i].out = r.out ? dout[i].get() : nullptr;
+        }
+        const sdfx_status st = grid_batch_core(dreq.data(), request_count, sign, s);
+        for (uint64_t i = 0; i < request_count; ++i) {
+            requests[i].status = dreq[i].status;
+            const uint64_t nc = requests[i].cell_count[0] * requests[i].cell_count[1] * requests[i].cell_count[2];
+            if (dreq[i].status == SDFX_OK && nc && requests[i].out)
+                SDFX_HIP(hipMemcpyAsync(requests[i].out, dout[i].get(), nc * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        SDFX_HIP(hipStreamSynchronize(s));
+        return st;
     });
 }
 

@@ -1,10 +1,12 @@
-// sdfx_grid.hip — generate_grid_sdf (generate/grid.rs:265-378) on the device.
+// sdfx_grid.hip — generate_grid_sdf (generate/grid.rs:265-378) on the device,
+// for one request or a batch of independent (mesh, grid) requests run through
+// the same launches.
 //
 // Cells hold a 64-bit key ordered like the reference's stored distance:
 //   Raycast: bits(d) << 32 | triangle        (d >= 0, so the bits order as floats)
 //   Normal:  bits(|d|) << 32 | negative << 31 | triangle
 // so one atomicMin keeps the smallest distance, then (Normal) the positive
-// one, then the lowest triangle index.
+// one, then the lowest triangle index (the index within the request's mesh).
 //
 // 1. Seeding (generate/grid.rs:383-457): every (triangle, cell) pair of each
 //    triangle's snapped, widened padded box, one lane per pair (the pair list
@@ -12,10 +14,10 @@
 //    lane).
 // 2. Propagation (generate/grid.rs:495-558): the reference's relaxation rule
 //    (a cell's triangle is tried on its neighbours and kept on strict
-//    improvement) run as rounds. Round r snapshots (cell, triangle) of every
-//    cell improved in round r-1, then tries each on its 26 neighbours; the
-//    snapshot makes each round, and so the fixpoint, independent of
-//    execution order.
+//    improvement) run as rounds. Round r takes the flagged cells of each
+//    request that lie in that request's distance band, snapshots (cell,
+//    triangle) and tries each on its 26 neighbours; the snapshot makes each
+//    round, and so the fixpoint, independent of execution order.
 // 3. Sign (generate/grid.rs:568-684): the rays start at the cells with index 0
 //    along +X/+Y/+Z. Instead of walking a BVH per ray, every triangle visits
 //    the rays whose origin lies in its padded box (the rays the reference's
@@ -24,8 +26,16 @@
 //    row gives the parity of hits at or beyond each cell, i.e. of the
 //    reference's `+1 on cells 0..=k` counters. The 2-of-3 vote is fused into
 //    the output write.
+//
+// Batches: requests share the launches but never each other's state. Each
+// request's cells start on a tile boundary, every per-round decision (its band
+// minimum, its previous round's size) reads only that request's words, and the
+// keys hold mesh-local triangle indices, so every result is bit-identical to
+// the same request run alone.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "sdfx_internal.hpp"
 
@@ -40,12 +50,44 @@ struct GridDev {
     uint32_t words[3];  // u32 words per ray row along each axis
 };
 
+// One request of a batch as the kernels see it.
+struct GridReq {
+    GridDev g;
+    uint64_t cell_off;      // first cell in the batch's key array (a multiple of COMPACT_TILE)
+    uint64_t tri_off;       // first triangle in the batch's triangle array
+    uint64_t ntris;
+    uint64_t row_off[3];    // first ray-parity word of each axis
+    uint64_t row_first[3];  // first ray row of each axis in the batch's row numbering
+    float* out;
+    uint32_t ncells;
+    float band;        // width of the distance band a round takes
+    uint32_t release;  // after a round that took fewer cells, a round takes every flagged cell
+    uint32_t flag;     // index of the request's DevFlags
+};
+
+// Request lookup. A batch reads its table; a single request travels by value
+// in the kernel arguments (scalar registers), which saves a dependent load on
+// every access. Kernels take MODE = sign | batch << 1.
+template <bool BATCH>
+struct ReqView {
+    const GridReq* table;
+    GridReq one;
+    __device__ __forceinline__ const GridReq& operator[](uint32_t q) const {
+        if constexpr (BATCH) return table[q];
+        else return one;
+    }
+};
+template <int MODE>
+constexpr int mode_sign = MODE & 1;
+template <int MODE>
+constexpr bool mode_batch = (MODE & 2) != 0;
+
 __device__ __forceinline__ float cell_coord(const GridDev& g, int k, uint32_t i) {
     return g.first[k] + (float)i * g.size[k];  // grid.rs:289-295
 }
 
-__device__ __forceinline__ uint64_t cell_index(const GridDev& g, uint32_t x, uint32_t y, uint32_t z) {
-    return (uint64_t)z + (uint64_t)y * g.n[2] + (uint64_t)x * g.n[1] * g.n[2];  // grid.rs:276-278
+__device__ __forceinline__ uint32_t cell_index(const GridDev& g, uint32_t x, uint32_t y, uint32_t z) {
+    return (x * g.n[1] + y) * g.n[2] + z;  // grid.rs:276-278 (below 2^32: checked on the host)
 }
 
 // `f32 as isize` (saturating, NaN -> 0) after floor, clamped to [0, n - 1]
@@ -92,13 +134,112 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
-__global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
 __global__ void k_fill_u32g(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// ---------------------------------------------------------------- round bookkeeping
+
+constexpr int COMPACT_PER_LANE = 16;
+constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
+// Width of the distance band a round takes, in units of the largest |cell size|.
+constexpr float GRID_BAND_CELLS = 3.0f;
+// A round after one that took fewer than max(GRID_RELEASE_MIN, cells /
+// GRID_RELEASE_DIV) cells takes every flagged cell: once the front is thin,
+// waiting for the band only adds rounds.
+constexpr uint64_t GRID_RELEASE_MIN = 1024;
+constexpr uint64_t GRID_RELEASE_DIV = 128;
+// Device-scope atomics on one address serialise (about 11 ns each on MI355X),
+// so shared words are spread: the item counters over NCLASS classes (tile t
+// belongs to class t * NCLASS / ntiles, a run of neighbouring tiles, so the
+// item buffer stays in spatial order), each request's band minimum and round
+// size over req_spread words (64 for a single request, which every block of a
+// launch folds into; 16 per request of a batch, whose blocks spread over the
+// requests), all WORD_STRIDE u32 (256 bytes) apart.
+constexpr uint32_t NCLASS = 64;
+template <int MODE>
+constexpr uint32_t req_spread = mode_batch<MODE> ? 16u : 64u;
+constexpr uint32_t WORD_STRIDE = 64;
+// Frontier item coordinates are packed as x | y << 21 | z << 42.
+constexpr uint32_t AXIS_XY_LIMIT = 1u << 21;
+constexpr uint32_t AXIS_Z_LIMIT = 1u << 22;
+
+// One frontier entry: a cell taken in this round and the mesh-local triangle
+// its key held when the round started. A single request stores the cell's
+// coordinates as they are, {tri, x, y, z}; a batch adds the request and packs
+// the coordinates, {tri, req, x | y << 21 | z << 42 (two words)}.
+struct FrontierItem {
+    uint32_t w[4];
+};
+
+template <int MODE>
+__device__ __forceinline__ FrontierItem make_item(uint32_t tri, uint32_t q, uint32_t x, uint32_t y, uint32_t z) {
+    if constexpr (mode_batch<MODE>) {
+        const uint64_t p = (uint64_t)x | ((uint64_t)y << 21) | ((uint64_t)z << 42);
+        return FrontierItem{{tri, q, (uint32_t)p, (uint32_t)(p >> 32)}};
+    } else {
+        return FrontierItem{{tri, x, y, z}};
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void read_item(const FrontierItem& it, uint32_t& tri, uint32_t& q, uint32_t& x, uint32_t& y,
+                                          uint32_t& z) {
+    tri = it.w[0];
+    if constexpr (mode_batch<MODE>) {
+        const uint64_t p = ((uint64_t)it.w[3] << 32) | it.w[2];
+        q = it.w[1];
+        x = (uint32_t)(p & (AXIS_XY_LIMIT - 1));
+        y = (uint32_t)((p >> 21) & (AXIS_XY_LIMIT - 1));
+        z = (uint32_t)(p >> 42);
+    } else {
+        q = 0;
+        x = it.w[1];
+        y = it.w[2];
+        z = it.w[3];
+    }
+}
+
+// Word s (taken modulo the spread) of request q in a per-request spread array.
+template <int MODE>
+__device__ __forceinline__ uint32_t* req_word(uint32_t* base, uint32_t q, uint32_t s) {
+    return base + ((uint64_t)q * req_spread<MODE> + s % req_spread<MODE>) * WORD_STRIDE;
+}
+
+// Minimum and sum over the spread words of request q, computed by every wave
+// on its own (lane l < req_spread holds word l).
+template <int MODE>
+__device__ __forceinline__ uint32_t req_min(const uint32_t* base, uint32_t q) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t m = lane < req_spread<MODE> ? base[((uint64_t)q * req_spread<MODE> + lane) * WORD_STRIDE] : UINT32_MAX;
+    for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    return m;
+}
+template <int MODE>
+__device__ __forceinline__ uint32_t req_sum(const uint32_t* base, uint32_t q) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t m = lane < req_spread<MODE> ? base[((uint64_t)q * req_spread<MODE> + lane) * WORD_STRIDE] : 0u;
+    for (int off = 32; off > 0; off >>= 1) m += (uint32_t)__shfl_xor((int)m, off, 64);
+    return m;
+}
+
+// Folds each lane's (request q, value m) into that request's minimum words;
+// UINT32_MAX means "nothing". A wave whose lanes share one request makes one
+// atomic (the usual case: work is laid out request by request). All lanes of
+// the wave must call it.
+template <int MODE>
+__device__ __forceinline__ void fold_req_min(uint32_t* base, uint32_t q, uint32_t m, uint32_t slot) {
+    const uint64_t has = __ballot(m != UINT32_MAX);
+    if (!has) return;
+    const uint32_t q0 = (uint32_t)__shfl((int)q, __ffsll((long long)has) - 1, 64);
+    if (m == UINT32_MAX) q = q0;
+    if (__all(q == q0)) {
+        for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off, 64));
+        if ((threadIdx.x & 63u) == 0) atomic_min_if_lower(req_word<MODE>(base, q0, slot), m);
+    } else if (m != UINT32_MAX) {
+        atomic_min_if_lower(req_word<MODE>(base, q, slot), m);
+    }
 }
 
 // ---------------------------------------------------------------- seeding
@@ -110,10 +251,19 @@ struct SeedBox {
 
 // generate/grid.rs:400-426: padded box, snapped corners, widened by one cell
 // where the snapped corner's centre lies beyond the box.
-__global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ tris, uint64_t nt, GridDev g,
+template <int MODE>
+__global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ tris, uint64_t nt,
+                                                    ReqView<mode_batch<MODE>> reqs,
+                                                    const uint32_t* __restrict__ tri_req,
                                                     SeedBox* __restrict__ boxes, uint64_t* __restrict__ counts) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
+    const uint32_t q = mode_batch<MODE> ? tri_req[t] : 0u;
+    if (q == UINT32_MAX) {  // a triangle of no request in this run
+        counts[t] = 0;
+        return;
+    }
+    const GridDev g = reqs[q].g;
     const TriRec tr = tris[t];
     Box b;
     tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
@@ -141,18 +291,20 @@ __global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ t
     counts[t] = cnt;
 }
 
-template <int SIGN>
-__global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ tris, uint64_t nt, GridDev g,
+template <int MODE>
+__global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ tris, uint64_t nt,
+                                                    ReqView<mode_batch<MODE>> reqs,
+                                                    const uint32_t* __restrict__ tri_req,
                                                     const SeedBox* __restrict__ boxes,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint64_t* __restrict__ total_ptr, uint64_t* keys,
                                                     DevFlags* flags, unsigned long long* evals) {
+    constexpr int SIGN = mode_sign<MODE>;
     const uint64_t total = *total_ptr;
-    bool nan_seen = false;
     uint32_t n_evals = 0;
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
          p += (uint64_t)gridDim.x * blockDim.x) {
-        // last triangle whose offset is <= p
+        // last triangle whose offset is <= p (the one whose pairs hold p)
         uint64_t lo = 0, hi = nt - 1;
         while (lo < hi) {
             uint64_t mid = (lo + hi + 1) >> 1;
@@ -160,11 +312,12 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
             else hi = mid - 1;
         }
         const uint64_t t = lo;
+        const GridReq& R = reqs[mode_batch<MODE> ? tri_req[t] : 0u];
         const SeedBox sb = boxes[t];
         uint64_t local = p - offsets[t];
         uint64_t yz = (uint64_t)sb.ext[1] * sb.ext[2];
         if (yz == 0 || local >= yz * sb.ext[0]) {  // cannot happen with a consistent prefix sum
-            atomicOr(&flags->internal, 1u);
+            atomicOr(&flags[R.flag].internal, 1u);
             continue;
         }
         uint32_t x = sb.mn[0] + (uint32_t)(local / yz);
@@ -172,126 +325,105 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
         uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
         uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
         const TriRec tr = tris[t];
-        float d = cell_dist<SIGN>(g, x, y, z, tr);
+        float d = cell_dist<SIGN>(R.g, x, y, z, tr);
         n_evals++;
-        nan_seen |= d != d;
+        if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
         if (key_valid(d)) {
-            uint64_t key = make_key<SIGN>(d, (uint32_t)t);
-            uint64_t* slot = &keys[cell_index(g, x, y, z)];
+            uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
+            uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
             if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
         }
     }
-    if (nan_seen) atomicOr(&flags->nan_distance, 1u);
     if (evals) block_count(evals, n_evals);
 }
 
 // ---------------------------------------------------------------- propagation
 
-constexpr int COMPACT_PER_LANE = 16;
-constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
-// Width of the distance band a round takes, in units of the largest |cell size|.
-constexpr float GRID_BAND_CELLS = 3.0f;
-// A round after one that took fewer than max(GRID_RELEASE_MIN, cells /
-// GRID_RELEASE_DIV) cells takes every flagged cell: once the front is thin,
-// waiting for the band only adds rounds.
-constexpr uint64_t GRID_RELEASE_MIN = 1024;
-constexpr uint64_t GRID_RELEASE_DIV = 128;
-// Round bookkeeping is spread over NCLASS words, WORD_STRIDE u32 (256 bytes)
-// apart: device-scope atomics on one address serialise (about 11 ns each on
-// MI355X), so tile t appends its items through the counter of its class
-// t * NCLASS / ntiles (a run of neighbouring tiles) into that class's region of
-// the item buffer, and block b folds band minima into word b % NCLASS. The
-// item buffer thus stays in tile order, i.e. in spatial order.
-constexpr uint32_t NCLASS = 64;
-constexpr uint32_t WORD_STRIDE = 64;
-
-// One frontier entry: a cell taken in this round, its coordinates and the
-// triangle its key held when the round started.
-struct FrontierItem {
-    uint32_t tri;
-    uint32_t x, y, z;
-};
-
-// Smallest of the NCLASS spread words, computed by every wave on its own.
-__device__ __forceinline__ uint32_t spread_min(const uint32_t* __restrict__ w) {
-    uint32_t m = w[(threadIdx.x & 63u) * WORD_STRIDE];  // NCLASS == 64: one word per lane
-    for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off, 64));
-    return m;
-}
-
-// Flags every seeded cell (round 0 takes the seeds) and folds the smallest
-// seeded distance into the spread minimum words.
+// Flags every seeded cell (round 0 takes the seeds) and folds each request's
+// smallest seeded distance into its minimum words.
+template <int MODE>
 __global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict__ keys, uint64_t ncells,
+                                                     const uint32_t* __restrict__ tile_req,
                                                      uint8_t* __restrict__ changed, uint8_t* __restrict__ tile_changed,
-                                                     uint64_t tile, uint32_t* __restrict__ min_out) {
-    uint32_t m = UINT32_MAX;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = keys[i];
-        const bool seeded = k != KEY_EMPTY;
-        changed[i] = seeded ? 1 : 0;
-        if (seeded) m = min(m, (uint32_t)(k >> 32));
-        if (i % tile == 0) tile_changed[i / tile] = 1;  // conservatively: every tile is scanned once
+                                                     uint32_t* __restrict__ min_out) {
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ncells; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;  // the loop bounds are the same for every lane
+        uint32_t m = UINT32_MAX, q = 0;
+        if (i < ncells) {
+            const uint64_t k = keys[i];
+            const bool seeded = k != KEY_EMPTY;
+            changed[i] = seeded ? 1 : 0;
+            if (mode_batch<MODE>) q = tile_req[i / COMPACT_TILE];
+            if (seeded) m = (uint32_t)(k >> 32);
+            if (i % COMPACT_TILE == 0) tile_changed[i / COMPACT_TILE] = 1;  // every tile is scanned once
+        }
+        if (mode_batch<MODE>) fold_req_min<MODE>(min_out, q, m, blockIdx.x);
+        else block_min_u32(req_word<MODE>(min_out, 0, blockIdx.x), m);
     }
-    block_min_u32(min_out + (blockIdx.x % NCLASS) * WORD_STRIDE, m);
 }
 
-// Snapshot of one round's frontier. The round takes every flagged cell whose
-// distance (the high word of its key, non-negative float bits) is at most
-// min + band, where min is the smallest distance among all flagged cells (the
-// minimum of the spread words min_cur, complete when the round starts), or
-// every flagged cell when the previous round took fewer than `release` cells
-// (*prev_taken); each taken cell becomes an item carrying the triangle of its
+// Snapshot of one round's frontier. Tile t belongs to request q; the round
+// takes every flagged cell of q whose distance (the high word of its key,
+// non-negative float bits) is at most min_q + band_q, where min_q is the
+// smallest distance among q's flagged cells (the minimum of q's words in
+// min_cur, complete when the round starts), or every flagged cell of q when
+// q's previous round took fewer than release_q cells (the sum of q's words in
+// taken_prev). Each taken cell becomes an item carrying the triangle of its
 // current key and its flag is cleared. Flagged cells beyond the band keep
-// their flags, and their smallest distance is folded into min_next for the
-// next round.
+// their flags; their smallest distance is folded into min_next and the number
+// of taken cells into taken_cur.
 //
 // Each block takes a contiguous tile of COMPACT_TILE cells (16 per lane, flags
 // read as one 16-byte vector), ranks the taken cells with a block-wide prefix
 // sum and reserves its output range in its class's region with a single
 // atomic, so items stay in cell order within a tile. Also resets the counters
 // the next compaction appends to.
-__global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __restrict__ keys, GridDev g,
+template <int MODE>
+__global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __restrict__ keys,
+                                                          ReqView<mode_batch<MODE>> reqs,
+                                                          const uint32_t* __restrict__ tile_req, uint32_t ntiles,
                                                           uint8_t* __restrict__ changed,
                                                           uint8_t* __restrict__ tile_changed,
                                                           FrontierItem* __restrict__ items, uint32_t region,
                                                           uint32_t* __restrict__ ctr_cur,
                                                           uint32_t* __restrict__ ctr_next,
                                                           const uint32_t* __restrict__ min_cur,
-                                                          uint32_t* __restrict__ min_next, float band,
-                                                          const uint32_t* __restrict__ prev_taken, uint32_t release,
-                                                          int sign) {
+                                                          uint32_t* __restrict__ min_next,
+                                                          const uint32_t* __restrict__ taken_prev,
+                                                          uint32_t* __restrict__ taken_cur, int sign) {
     __shared__ uint32_t wave_tot[4], wave_min[4];
     __shared__ uint32_t block_base;
-    const uint32_t ncells = g.n[0] * g.n[1] * g.n[2];  // below 2^32 (checked on the host)
-    const uint32_t ntiles = (ncells + COMPACT_TILE - 1) / COMPACT_TILE;
     if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
-    const float mf = bitsf(spread_min(min_cur));
-    float limit = mf + band;
-    // a NaN band (or no flagged cell) or a thin previous front: every flagged cell
-    if (!(limit >= mf) || *prev_taken < release) limit = INFINITY;
-    const uint32_t lbits = fbits(limit);  // non-negative floats order as their bits
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    for (uint32_t tile = blockIdx.x * (uint32_t)COMPACT_TILE; tile < ncells; tile += gridDim.x * (uint32_t)COMPACT_TILE) {
-        const uint32_t tile_id = tile / COMPACT_TILE;
-        const uint32_t c0 = tile + threadIdx.x * COMPACT_PER_LANE;
-        // the tile flag and this lane's 16 cell flags are read together
+    for (uint32_t tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
+        const uint32_t q = mode_batch<MODE> ? tile_req[tile_id] : 0u;
+        const GridReq& R = reqs[q];
+        const uint32_t local0 = (uint32_t)((uint64_t)tile_id * COMPACT_TILE - R.cell_off);
+        const uint32_t c0 = local0 + threadIdx.x * COMPACT_PER_LANE;  // first cell of this lane, request-local
+        const uint64_t g0 = R.cell_off + c0;                           // the same cell in the batch
+        // the tile flag, this lane's 16 cell flags and the band words are read together
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
         const uint8_t tflag = tile_changed[tile_id];
-        if (c0 < ncells) v = *reinterpret_cast<const uint4*>(changed + c0);  // allocation padded to 16 bytes
+        if (c0 < R.ncells) v = *reinterpret_cast<const uint4*>(changed + g0);  // whole tiles are allocated
+        const float mf = bitsf(req_min<MODE>(min_cur, q));
+        const uint32_t prev = req_sum<MODE>(taken_prev, q);
         if (!tflag) continue;  // block-uniform: no flagged cell in the tile
+        float limit = mf + R.band;
+        // a NaN band (or no flagged cell) or a thin previous front: every flagged cell
+        if (!(limit >= mf) || prev < R.release) limit = INFINITY;
+        const uint32_t lbits = fbits(limit);  // non-negative floats order as their bits
         uint32_t flagged = 0;  // bit k: cell c0 + k is flagged
         {
             const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int k = 0; k < COMPACT_PER_LANE; ++k) flagged |= ((w[k >> 2] >> (8 * (k & 3))) & 1u) << k;
-            if (c0 >= ncells) flagged = 0;
-            else if (ncells - c0 < (uint32_t)COMPACT_PER_LANE) flagged &= (1u << (ncells - c0)) - 1u;
+            if (c0 >= R.ncells) flagged = 0;
+            else if (R.ncells - c0 < (uint32_t)COMPACT_PER_LANE) flagged &= (1u << (R.ncells - c0)) - 1u;
         }
         // every flagged cell's key first, so the loads are in flight together
         uint64_t kv[COMPACT_PER_LANE];
 #pragma unroll
-        for (int k = 0; k < COMPACT_PER_LANE; ++k) kv[k] = ((flagged >> k) & 1u) ? keys[c0 + k] : KEY_EMPTY;
+        for (int k = 0; k < COMPACT_PER_LANE; ++k) kv[k] = ((flagged >> k) & 1u) ? keys[g0 + k] : KEY_EMPTY;
         uint32_t take = 0, dmin = UINT32_MAX;
 #pragma unroll
         for (int k = 0; k < COMPACT_PER_LANE; ++k) {
@@ -317,17 +449,19 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
         __syncthreads();
         uint32_t before = 0, total = 0, bmin = UINT32_MAX;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            before += (uint32_t)q < wave ? wave_tot[q] : 0u;
-            total += wave_tot[q];
-            bmin = min(bmin, wave_min[q]);
+        for (int qw = 0; qw < 4; ++qw) {
+            before += (uint32_t)qw < wave ? wave_tot[qw] : 0u;
+            total += wave_tot[qw];
+            bmin = min(bmin, wave_min[qw]);
         }
         const uint32_t cls = (uint32_t)((uint64_t)tile_id * NCLASS / ntiles);
         if (threadIdx.x == 0) block_base = total ? atomicAdd(ctr_cur + cls * WORD_STRIDE, total) : 0u;
         __syncthreads();
         if (threadIdx.x == 0) {
             tile_changed[tile_id] = bmin != UINT32_MAX ? 1 : 0;  // deferred cells keep the tile flagged
-            if (bmin != UINT32_MAX) atomicMin(min_next + cls * WORD_STRIDE, bmin);  // nothing waits for it
+            // nothing waits for these
+            if (bmin != UINT32_MAX) atomicMin(req_word<MODE>(min_next, q, tile_id), bmin);
+            if (total) atomicAdd(req_word<MODE>(taken_cur, q, tile_id), total);
         }
         if (take) {
             uint32_t slot = cls * region + block_base + before + x - mine;
@@ -335,14 +469,14 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
             for (int k = 0; k < COMPACT_PER_LANE; ++k) {
                 if (!((take >> k) & 1u)) continue;
                 const uint32_t i = c0 + k;
-                const uint32_t z = i % g.n[2];
-                const uint32_t r = i / g.n[2];
-                items[slot++] = FrontierItem{key_tri(kv[k], sign), r / g.n[1], r % g.n[1], z};
+                const uint32_t z = i % R.g.n[2];
+                const uint32_t r = i / R.g.n[2];
+                items[slot++] = make_item<MODE>(key_tri(kv[k], sign), q, r / R.g.n[1], r % R.g.n[1], z);
             }
             uint32_t w[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int k = 0; k < COMPACT_PER_LANE; ++k) w[k >> 2] |= ((deferred >> k) & 1u) << (8 * (k & 3));
-            *reinterpret_cast<uint4*>(changed + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+            *reinterpret_cast<uint4*>(changed + g0) = make_uint4(w[0], w[1], w[2], w[3]);
         }
         __syncthreads();  // wave_tot / wave_min / block_base are reused by the next tile
     }
@@ -355,9 +489,11 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
 constexpr uint32_t PROP_QUEUE = 128;  // a fill adds at most 64 to at most 63 left over
 constexpr uint32_t PROP_WAVES = 4;    // 256-thread blocks
 
+template <bool BATCH>
 struct PropQueue {
     uint32_t x[PROP_WAVES][PROP_QUEUE], y[PROP_WAVES][PROP_QUEUE], z[PROP_WAVES][PROP_QUEUE];
-    uint32_t tri[PROP_WAVES][PROP_QUEUE], cur_lo[PROP_WAVES][PROP_QUEUE], cur_hi[PROP_WAVES][PROP_QUEUE];
+    uint32_t tri[PROP_WAVES][PROP_QUEUE], req[PROP_WAVES][BATCH ? PROP_QUEUE : 1];
+    uint32_t cur_lo[PROP_WAVES][PROP_QUEUE], cur_hi[PROP_WAVES][PROP_QUEUE];
 };
 
 // LDS traffic between lanes of one wave: keep the compiler from moving the
@@ -368,20 +504,24 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-lane running minimum of improved distances for one request at a time.
 struct PropAcc {
-    uint32_t n_evals = 0, imin = UINT32_MAX;
-    bool nan_seen = false;
+    uint32_t n_evals = 0, imin = UINT32_MAX, req = 0;
 };
 
-// Evaluates one queued pair: the reference's distance of cell (x, y, z) to the
-// triangle, kept if its key is below the neighbour's current key.
-template <int SIGN>
-__device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const GridDev& g, uint64_t* keys,
-                                          uint8_t* __restrict__ changed, uint8_t* __restrict__ tile_changed,
-                                          uint32_t x, uint32_t y, uint32_t z, uint32_t tri, uint64_t cur,
-                                          PropAcc& acc) {
-    const TriRec tr = tris[tri];
-    const float px = cell_coord(g, 0, x), py = cell_coord(g, 1, y), pz = cell_coord(g, 2, z);
+// Evaluates one queued pair: the reference's distance of cell (x, y, z) of
+// request q to its mesh-local triangle tri, kept if its key is below the
+// neighbour's current key.
+template <int MODE>
+__device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const ReqView<mode_batch<MODE>>& reqs,
+                                          uint64_t* keys, uint8_t* __restrict__ changed,
+                                          uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
+                                          DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
+                                          uint32_t q, uint64_t cur, PropAcc& acc) {
+    constexpr int SIGN = mode_sign<MODE>;
+    const GridReq& R = reqs[q];
+    const TriRec tr = tris[R.tri_off + tri];
+    const float px = cell_coord(R.g, 0, x), py = cell_coord(R.g, 1, y), pz = cell_coord(R.g, 2, z);
     float dx, dy, dz, cx, cy, cz;
     closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
@@ -401,15 +541,20 @@ __device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const
     const float dist = sqrtf(d2);
     float d = dist;
     if (SIGN == SDFX_SIGN_NORMAL) d = dot3(dx, dy, dz, tr.nx, tr.ny, tr.nz) > 0.0f ? dist : -dist;
-    acc.nan_seen |= d != d;
+    if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
     if (!key_valid(d)) return;
     const uint64_t key = make_key<SIGN>(d, tri);
     if (key >= cur) return;
-    const uint32_t ci = (x * g.n[1] + y) * g.n[2] + z;
+    const uint64_t ci = R.cell_off + cell_index(R.g, x, y, z);
     const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
     if (key < old) {
         changed[ci] = 1;
         tile_changed[ci / COMPACT_TILE] = 1;
+        if (mode_batch<MODE> && q != acc.req && acc.imin != UINT32_MAX) {  // the lane moved on to another request
+            atomic_min_if_lower(req_word<MODE>(min_next, acc.req, blockIdx.x), acc.imin);
+            acc.imin = UINT32_MAX;
+        }
+        if (mode_batch<MODE>) acc.req = q;
         acc.imin = min(acc.imin, (uint32_t)(key >> 32));
     }
 }
@@ -419,24 +564,27 @@ __device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const
 // wave's queue to prop_eval. A strictly smaller key is stored with atomicMin
 // and flags the cell for a later round. The reads before the atomic are plain
 // (cacheable) loads: keys only decrease, so a stale value can only cost an
-// unneeded evaluation or atomic. The smallest improved distance is folded into
-// min_next; min_cur (read by this round's compaction, written by the next one)
-// is reset to "none". The frontier size goes to *total_out for the next round's
-// compaction and, when host_total is not null, to that pinned host word.
-template <int SIGN>
-__global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
+// unneeded evaluation or atomic. Improved distances are folded into each
+// request's min_next words; min_cur and taken_next (read by this round's
+// compaction, written by the next one) are reset. The frontier size goes to
+// *total_out and, when host_total is not null, to that pinned host word.
+template <int MODE>
+__global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
                                                             const uint32_t* __restrict__ ctr_cur,
-                                                            const TriRec* __restrict__ tris, GridDev g,
+                                                            const TriRec* __restrict__ tris,
+                                                            ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
                                                             uint64_t* keys, uint8_t* __restrict__ changed,
                                                             uint8_t* __restrict__ tile_changed,
                                                             uint32_t* __restrict__ min_next,
                                                             uint32_t* __restrict__ min_cur,
+                                                            uint32_t* __restrict__ taken_next,
                                                             uint32_t* __restrict__ total_out,
                                                             uint32_t* __restrict__ host_total, DevFlags* flags,
                                                             unsigned long long* evals) {
+    constexpr int SIGN = mode_sign<MODE>;
     // prefix[c]: items in the classes before c
     __shared__ uint32_t prefix[NCLASS + 1];
-    __shared__ PropQueue Q;
+    __shared__ PropQueue<mode_batch<MODE>> Q;
     if (threadIdx.x < 64u) {
         const uint32_t c = ctr_cur[threadIdx.x * WORD_STRIDE];
         uint32_t x = c;
@@ -447,16 +595,18 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
         }
         prefix[threadIdx.x + 1] = x;
         if (threadIdx.x == 0) prefix[0] = 0;
-        if (blockIdx.x == 0) {
-            min_cur[threadIdx.x * WORD_STRIDE] = UINT32_MAX;
-            if (threadIdx.x == 63) {
-                *total_out = x;
-                if (host_total) {
-                    *host_total = x;
-                    __threadfence_system();
-                }
+        if (blockIdx.x == 0 && threadIdx.x == 63) {
+            *total_out = x;
+            if (host_total) {
+                *host_total = x;
+                __threadfence_system();
             }
         }
+    }
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < (uint64_t)nreq * req_spread<MODE>;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        min_cur[w * WORD_STRIDE] = UINT32_MAX;
+        taken_next[w * WORD_STRIDE] = 0u;
     }
     __syncthreads();
     const uint32_t work = prefix[NCLASS] * 26u;  // below 2^32: the frontier holds fewer than 2^32 / 26 cells
@@ -472,7 +622,7 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
     for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
         const uint32_t j = base + lane;
         bool keep = false;
-        uint32_t x = 0, y = 0, z = 0, tri = 0;
+        uint32_t x = 0, y = 0, z = 0, tri = 0, q = 0;
         uint64_t cur = 0;
         if (j < j1) {
             const uint32_t f = j / 26u;
@@ -482,13 +632,14 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
 #pragma unroll
             for (uint32_t step = NCLASS / 2; step; step >>= 1)
                 if (prefix[cls + step] <= f) cls += step;
-            const FrontierItem it = items[cls * region + (f - prefix[cls])];
-            x = it.x + k / 9u - 1u;
-            y = it.y + (k / 3u) % 3u - 1u;
-            z = it.z + k % 3u - 1u;
-            tri = it.tri;
-            if (x < g.n[0] && y < g.n[1] && z < g.n[2]) {  // -1 wraps to 2^32 - 1
-                cur = keys[(x * g.n[1] + y) * g.n[2] + z];  // grid.rs:276-278
+            uint32_t ix, iy, iz;
+            read_item<MODE>(items[cls * region + (f - prefix[cls])], tri, q, ix, iy, iz);
+            x = ix + k / 9u - 1u;
+            y = iy + (k / 3u) % 3u - 1u;
+            z = iz + k % 3u - 1u;
+            const GridReq& R = reqs[q];
+            if (x < R.g.n[0] && y < R.g.n[1] && z < R.g.n[2]) {  // -1 wraps to 2^32 - 1
+                cur = keys[R.cell_off + cell_index(R.g, x, y, z)];
                 // A neighbour already holding this triangle holds exactly the
                 // key the triangle would give it, and keys only decrease.
                 keep = cur == KEY_EMPTY || key_tri(cur, SIGN) != tri;
@@ -501,6 +652,7 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
             Q.y[wave][pos] = y;
             Q.z[wave][pos] = z;
             Q.tri[wave][pos] = tri;
+            if constexpr (mode_batch<MODE>) Q.req[wave][pos] = q;
             Q.cur_lo[wave][pos] = (uint32_t)cur;
             Q.cur_hi[wave][pos] = (uint32_t)(cur >> 32);
         }
@@ -510,18 +662,19 @@ __global__ void __launch_bounds__(256) k_frontier_propagate(const FrontierItem* 
             qn -= 64u;
             const uint32_t e = qn + lane;
             const uint64_t c = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
-            prop_eval<SIGN>(tris, g, keys, changed, tile_changed, Q.x[wave][e], Q.y[wave][e], Q.z[wave][e],
-                            Q.tri[wave][e], c, acc);
+            prop_eval<MODE>(tris, reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][e], Q.y[wave][e],
+                            Q.z[wave][e], Q.tri[wave][e], mode_batch<MODE> ? Q.req[wave][e] : 0u, c, acc);
             wave_lds_sync();
         }
     }
     if (lane < qn) {
         const uint64_t c = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
-        prop_eval<SIGN>(tris, g, keys, changed, tile_changed, Q.x[wave][lane], Q.y[wave][lane], Q.z[wave][lane],
-                        Q.tri[wave][lane], c, acc);
+        prop_eval<MODE>(tris, reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][lane],
+                        Q.y[wave][lane], Q.z[wave][lane], Q.tri[wave][lane],
+                        mode_batch<MODE> ? Q.req[wave][lane] : 0u, c, acc);
     }
-    if (acc.nan_seen) atomicOr(&flags->nan_distance, 1u);
-    block_min_u32(min_next + (blockIdx.x % NCLASS) * WORD_STRIDE, acc.imin);
+    if (mode_batch<MODE>) fold_req_min<MODE>(min_next, acc.req, acc.imin, blockIdx.x);
+    else block_min_u32(req_word<MODE>(min_next, 0, blockIdx.x), acc.imin);
     if (evals) block_count(evals, acc.n_evals);
 }
 
@@ -577,27 +730,44 @@ __device__ void scatter_axis(const TriRec& tr, const Box& b, const GridDev& g, u
     }
 }
 
-__global__ void __launch_bounds__(256) k_ray_scatter(const TriRec* __restrict__ tris, uint64_t nt, GridDev g,
+template <int MODE>
+__global__ void __launch_bounds__(256) k_ray_scatter(const TriRec* __restrict__ tris, uint64_t nt,
+                                                     ReqView<mode_batch<MODE>> reqs,
+                                                     const uint32_t* __restrict__ tri_req,
                                                      uint32_t* __restrict__ bx, uint32_t* __restrict__ by,
                                                      uint32_t* __restrict__ bz) {
     uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= 3 * nt) return;
     uint64_t t = id / 3;
     int axis = (int)(id % 3);
+    const uint32_t q = mode_batch<MODE> ? tri_req[t] : 0u;
+    if (q == UINT32_MAX) return;  // a triangle of no request in this run
+    const GridReq& R = reqs[q];
     const TriRec tr = tris[t];
     Box b;
     tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
-    if (axis == 0) scatter_axis<0>(tr, b, g, bx);
-    else if (axis == 1) scatter_axis<1>(tr, b, g, by);
-    else scatter_axis<2>(tr, b, g, bz);
+    if (axis == 0) scatter_axis<0>(tr, b, R.g, bx + R.row_off[0]);
+    else if (axis == 1) scatter_axis<1>(tr, b, R.g, by + R.row_off[1]);
+    else scatter_axis<2>(tr, b, R.g, bz + R.row_off[2]);
 }
 
 // Per row: bit i becomes the XOR of flip bits i..n-1 (the parity of hits whose
-// last counted cell is at or beyond i).
-__global__ void __launch_bounds__(256) k_ray_suffix(uint32_t* __restrict__ bits, uint64_t rows, uint32_t words) {
+// last counted cell is at or beyond i). Rows of all requests along one axis are
+// numbered consecutively; a row finds its request by binary search.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_ray_suffix(uint32_t* __restrict__ bits, uint64_t rows,
+                                                    ReqView<mode_batch<MODE>> reqs, uint32_t nreq, int axis) {
     uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
-    uint32_t* row = bits + r * words;
+    uint32_t lo = 0, hi = nreq - 1;  // the last request whose first row is <= r
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].row_first[axis] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    const GridReq& R = reqs[lo];
+    const uint32_t words = R.g.words[axis];
+    uint32_t* row = bits + R.row_off[axis] + (r - R.row_first[axis]) * words;
     uint32_t carry = 0;
     for (int w = (int)words - 1; w >= 0; --w) {
         uint32_t x = row[w];
@@ -612,34 +782,88 @@ __global__ void __launch_bounds__(256) k_ray_suffix(uint32_t* __restrict__ bits,
     }
 }
 
-template <int SIGN>
-__global__ void __launch_bounds__(256) k_grid_finish(const uint64_t* __restrict__ keys, GridDev g,
+template <int MODE>
+__global__ void __launch_bounds__(256) k_grid_finish(const uint64_t* __restrict__ keys, uint64_t ncells,
+                                                     ReqView<mode_batch<MODE>> reqs,
+                                                     const uint32_t* __restrict__ tile_req,
                                                      const uint32_t* __restrict__ px,
                                                      const uint32_t* __restrict__ py,
-                                                     const uint32_t* __restrict__ pz, float* __restrict__ out) {
-    const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
+                                                     const uint32_t* __restrict__ pz) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
          i += (uint64_t)gridDim.x * blockDim.x) {
+        const GridReq& R = reqs[mode_batch<MODE> ? tile_req[i / COMPACT_TILE] : 0u];
+        const uint64_t l = i - R.cell_off;
+        if (l >= R.ncells) continue;  // tile padding after the request's last cell
+        const GridDev& g = R.g;
+        constexpr int SIGN = mode_sign<MODE>;
         float v = key_value(keys[i], SIGN);
         if (SIGN == SDFX_SIGN_RAYCAST) {
-            uint32_t z = (uint32_t)(i % g.n[2]);
-            uint32_t y = (uint32_t)((i / g.n[2]) % g.n[1]);
-            uint32_t x = (uint32_t)(i / ((uint64_t)g.n[1] * g.n[2]));
-            uint32_t ox = (px[((uint64_t)y * g.n[2] + z) * g.words[0] + (x >> 5)] >> (x & 31u)) & 1u;
-            uint32_t oy = (py[((uint64_t)x * g.n[2] + z) * g.words[1] + (y >> 5)] >> (y & 31u)) & 1u;
-            uint32_t oz = (pz[((uint64_t)x * g.n[1] + y) * g.words[2] + (z >> 5)] >> (z & 31u)) & 1u;
+            uint32_t z = (uint32_t)(l % g.n[2]);
+            uint32_t y = (uint32_t)((l / g.n[2]) % g.n[1]);
+            uint32_t x = (uint32_t)(l / ((uint64_t)g.n[1] * g.n[2]));
+            uint32_t ox = (px[R.row_off[0] + ((uint64_t)y * g.n[2] + z) * g.words[0] + (x >> 5)] >> (x & 31u)) & 1u;
+            uint32_t oy = (py[R.row_off[1] + ((uint64_t)x * g.n[2] + z) * g.words[1] + (y >> 5)] >> (y & 31u)) & 1u;
+            uint32_t oz = (pz[R.row_off[2] + ((uint64_t)x * g.n[1] + y) * g.words[2] + (z >> 5)] >> (z & 31u)) & 1u;
             if (ox + oy + oz >= 2u) v = -v;  // generate/grid.rs:633-638
         }
-        out[i] = v;
+        R.out[l] = v;
     }
 }
 
 // ---------------------------------------------------------------- driver
 
+// Owner request of every tile and every triangle of a batch (UINT32_MAX for a
+// triangle of no request in the run), by binary search over the table.
+__global__ void __launch_bounds__(256) k_batch_owners(const GridReq* __restrict__ reqs, uint32_t nreq,
+                                                      uint32_t* __restrict__ tile_req, uint64_t ntiles,
+                                                      uint32_t* __restrict__ tri_req, uint64_t nt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntiles) {
+        uint32_t lo = 0, hi = nreq - 1;  // the last request whose first tile is <= i
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (reqs[mid].cell_off / COMPACT_TILE <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        tile_req[i] = lo;
+    }
+    if (i < nt) {
+        uint32_t lo = 0, hi = nreq - 1;  // the last request whose first triangle is <= i
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (reqs[mid].tri_off <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        tri_req[i] = (reqs[lo].tri_off <= i && i < reqs[lo].tri_off + reqs[lo].ntris) ? lo : UINT32_MAX;
+    }
+}
+
+// A per-thread pinned buffer for small host-to-device uploads, so they are
+// asynchronous copies. A call reuses it only after its previous call has
+// synchronised the stream.
+static void* pinned_stage(size_t bytes) {
+    struct Stage {
+        void* p = nullptr;
+        size_t cap = 0;
+        ~Stage() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    static thread_local Stage st;
+    if (bytes > st.cap) {
+        if (st.p) SDFX_HIP(hipHostFree(st.p));
+        st.p = nullptr;
+        st.cap = 0;
+        SDFX_HIP(hipHostMalloc(&st.p, bytes, hipHostMallocDefault));
+        st.cap = bytes;
+    }
+    return st.p;
+}
+
 // Two pinned words per thread and their events: the last propagation of a
-// batch stores its frontier size straight into one of the words (no copy
-// kernel), and the host reads it after the batch's event while the next batch
-// runs.
+// batch of rounds stores its frontier size straight into one of the words (no
+// copy kernel), and the host reads it after that batch's event while the next
+// batch of rounds runs.
 struct BatchProbe {
     struct Slots {
         uint32_t* words = nullptr;    // host address
@@ -676,11 +900,69 @@ static unsigned stride_blocks(uint64_t n) {
     return (unsigned)(b ? b : 1);
 }
 
-template <int SIGN>
-static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float* d_out, DevFlags* d_flags,
+template <int MODE>
+static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFlags* d_flags,
                      Arena& A) {
+    constexpr int SIGN = mode_sign<MODE>;
     hipStream_t s = A.stream;
-    const uint64_t ncells = (uint64_t)g.n[0] * g.n[1] * g.n[2];
+    const uint32_t nreq = (uint32_t)jobs.size();
+    // request table: cells padded to whole tiles, rows numbered per axis
+    std::vector<GridReq> reqs(nreq);
+    uint64_t ncells = 0, rows_total[3] = {0, 0, 0}, words_total[3] = {0, 0, 0}, max_extent = 0;
+    for (uint32_t q = 0; q < nreq; ++q) {
+        const GridBatchItem& J = jobs[q];
+        GridReq& R = reqs[q];
+        for (int k = 0; k < 3; ++k) {
+            R.g.first[k] = J.gp.first[k];
+            R.g.size[k] = J.gp.size[k];
+            R.g.n[k] = (uint32_t)J.gp.count[k];
+            R.g.words[k] = (uint32_t)((J.gp.count[k] + 31) / 32);
+        }
+        if (J.gp.count[0] > AXIS_XY_LIMIT || J.gp.count[1] > AXIS_XY_LIMIT || J.gp.count[2] > AXIS_Z_LIMIT)
+            throw Error(SDFX_ERR_INVALID_ARGUMENT, "grid axes are limited to 2^21 (x, y) and 2^22 (z) cells");
+        const uint64_t nc = J.gp.count[0] * J.gp.count[1] * J.gp.count[2];
+        R.cell_off = ncells;
+        R.ncells = (uint32_t)nc;
+        R.tri_off = J.tri_off;
+        R.ntris = J.n_tris;
+        R.out = J.d_out;
+        R.flag = J.flag;
+        R.band = GRID_BAND_CELLS * std::fmax(std::fmax(std::fabs(R.g.size[0]), std::fabs(R.g.size[1])),
+                                             std::fabs(R.g.size[2]));
+        R.release = (uint32_t)std::max<uint64_t>(GRID_RELEASE_MIN, nc / GRID_RELEASE_DIV);
+        const uint64_t rows[3] = {J.gp.count[1] * J.gp.count[2], J.gp.count[0] * J.gp.count[2],
+                                  J.gp.count[0] * J.gp.count[1]};
+        for (int a = 0; a < 3; ++a) {
+            R.row_first[a] = rows_total[a];
+            R.row_off[a] = words_total[a];
+            rows_total[a] += rows[a];
+            words_total[a] += rows[a] * R.g.words[a];
+        }
+        const uint64_t tiles = (nc + COMPACT_TILE - 1) / COMPACT_TILE;
+        ncells += tiles * COMPACT_TILE;
+        max_extent = std::max<uint64_t>(max_extent, J.gp.count[0] + J.gp.count[1] + J.gp.count[2]);
+        // 32-bit cell and work indices: 26 neighbour slots per cell plus one
+        // grid stride stay below 2^32
+        if (ncells * 26 + (1ull << 24) >= (1ull << 32))
+            throw Error(SDFX_ERR_INVALID_ARGUMENT, "grids are limited to 165,000,000 cells per launch");
+    }
+    const uint32_t ntiles = (uint32_t)(ncells / COMPACT_TILE);
+    // a single request travels in the kernel arguments; a batch gets its table,
+    // and the owner request of every tile and triangle, on the device
+    GridReq* d_reqs = nullptr;
+    uint32_t *tile_req = nullptr, *tri_req = nullptr;
+    if (mode_batch<MODE>) {
+        d_reqs = A.alloc<GridReq>(nreq);
+        tile_req = A.alloc<uint32_t>(ntiles);
+        tri_req = A.alloc<uint32_t>(nt);
+        void* stage = pinned_stage(nreq * sizeof(GridReq));
+        std::memcpy(stage, reqs.data(), nreq * sizeof(GridReq));
+        SDFX_HIP(hipMemcpyAsync(d_reqs, stage, nreq * sizeof(GridReq), hipMemcpyHostToDevice, s));
+        SDFX_LAUNCH(k_batch_owners, dim3(blocks_for(std::max<uint64_t>(ntiles, nt), 256)), dim3(256), 0, s, d_reqs,
+                    nreq, tile_req, (uint64_t)ntiles, tri_req, nt);
+    }
+    ReqView<mode_batch<MODE>> rv{d_reqs, reqs[0]};
+
     uint64_t* keys = A.alloc<uint64_t>(ncells);
     SDFX_LAUNCH(k_fill_u64, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, KEY_EMPTY);
 
@@ -688,7 +970,8 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     SeedBox* boxes = A.alloc<SeedBox>(nt);
     uint64_t* counts = A.alloc<uint64_t>(nt);
     uint64_t* total = A.alloc<uint64_t>(1);
-    SDFX_LAUNCH(k_seed_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, g, boxes, counts);
+    SDFX_LAUNCH(k_seed_boxes<MODE>, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, rv, tri_req, boxes,
+                counts);
     exclusive_scan_u64(counts, counts, nt, total, A);
     // distance-evaluation counters, only while kernel timing is on
     // per-block slots: seeding (8192 blocks), then propagation (4096 blocks)
@@ -698,58 +981,58 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
         evals = A.alloc<unsigned long long>(seed_slots + prop_slots);
         fill_u32(reinterpret_cast<uint32_t*>(evals), 2 * (seed_slots + prop_slots), 0u, s);
     }
-    SDFX_LAUNCH(k_seed_pairs<SIGN>, dim3(8192), dim3(256), 0, s, d_tris, nt, g, boxes, counts, total, keys,
-                       d_flags, evals);
+    SDFX_LAUNCH(k_seed_pairs<MODE>, dim3(8192), dim3(256), 0, s, d_tris, nt, rv, tri_req, boxes, counts, total,
+                keys, d_flags, evals);
     SDFX_HIP(hipGetLastError());
 
     // 2. frontier rounds
-    // 32-bit cell and work indices: 26 neighbour slots per cell plus one grid
-    // stride stay below 2^32
-    if (ncells * 26 + (1ull << 24) >= (1ull << 32))
-        throw Error(SDFX_ERR_INVALID_ARGUMENT, "grid too large for one frontier (at most 165,000,000 cells)");
-    const uint64_t ntiles = (ncells + COMPACT_TILE - 1) / COMPACT_TILE;
-    // per parity of the round: NCLASS item counters, NCLASS band minima; then
-    // the two frontier totals
-    const uint64_t spread = (uint64_t)NCLASS * WORD_STRIDE;
-    uint32_t* words = A.alloc<uint32_t>(4 * spread + 2);
-    uint32_t* ctr[2] = {words, words + spread};
-    uint32_t* minw[2] = {words + 2 * spread, words + 3 * spread};
-    uint32_t* totals = words + 4 * spread;
+    // per parity of the round: NCLASS item counters, then per request its band
+    // minima and its taken-cell counts; then the two frontier totals
+    const uint64_t cls_words = (uint64_t)NCLASS * WORD_STRIDE;
+    const uint64_t req_words = (uint64_t)nreq * req_spread<MODE> * WORD_STRIDE;
+    uint32_t* words = A.alloc<uint32_t>(2 * cls_words + 4 * req_words + 2);
+    uint32_t* ctr[2] = {words, words + cls_words};
+    uint32_t* minw[2] = {words + 2 * cls_words, words + 2 * cls_words + req_words};
+    uint32_t* taken[2] = {words + 2 * cls_words + 2 * req_words, words + 2 * cls_words + 3 * req_words};
+    uint32_t* totals = words + 2 * cls_words + 4 * req_words;
     const uint32_t region = (uint32_t)((ntiles + NCLASS - 1) / NCLASS * COMPACT_TILE);  // items per class
-    uint8_t* changed = A.alloc<uint8_t>((ncells + 15) & ~15ull);  // 16-byte vector reads
+    uint8_t* changed = A.alloc<uint8_t>(ncells);  // whole tiles: 16-byte vector reads stay inside
     uint8_t* tile_changed = A.alloc<uint8_t>(ntiles);
     FrontierItem* items = A.alloc<FrontierItem>((uint64_t)region * NCLASS);
-    fill_u32(ctr[0], 2 * spread, 0u, s);
-    fill_u32(minw[0], 2 * spread, UINT32_MAX, s);
-    fill_u32(totals, 2, UINT32_MAX, s);  // round 0 reads totals[1]: "no thin front before it"
-    SDFX_LAUNCH(k_mark_seeded, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, changed, tile_changed,
-                (uint64_t)COMPACT_TILE, minw[0]);
+    fill_u32(ctr[0], 2 * cls_words, 0u, s);
+    fill_u32(minw[0], 2 * req_words, UINT32_MAX, s);
+    fill_u32(taken[0], req_words, 0u, s);
+    // round 0 reads taken[1] as its previous round: large, so the band applies
+    // (2^24 per word: the sum over a request's words, at least 2^28, exceeds
+    // every release threshold and stays below 2^32)
+    SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(req_words)), dim3(256), 0, s, taken[1], req_words, 1u << 24);
+    fill_u32(totals, 2, 0u, s);
+    SDFX_LAUNCH(k_mark_seeded<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, tile_req, changed,
+                tile_changed, minw[0]);
     SDFX_HIP(hipGetLastError());
-    const float band =
-        GRID_BAND_CELLS * std::fmax(std::fmax(std::fabs(g.size[0]), std::fabs(g.size[1])), std::fabs(g.size[2]));
-    const uint32_t release = (uint32_t)std::max<uint64_t>(GRID_RELEASE_MIN, ncells / GRID_RELEASE_DIV);
     const unsigned prop_blocks = 4096;
-    const unsigned compact_blocks = (unsigned)std::min<uint64_t>(ntiles, 4096);
+    const unsigned compact_blocks = std::min<uint32_t>(ntiles, 4096u);
     int batch = 8;
-    // Every round takes at least the nearest flagged cell and every improvement
-    // strictly lowers a key, so the loop ends; the cap only turns an
-    // inconsistency into an error instead of a hang.
-    const uint64_t max_rounds = 256ull * (g.n[0] + g.n[1] + g.n[2]) + 4096;
-    // Batches are enqueued one ahead of the one being checked: while the host
-    // waits for batch k's frontier size, batch k+1 is already on the GPU. Once
-    // the front thins out the batches shrink, so fewer empty rounds run after
-    // the last one that took a cell.
+    // Every round takes at least the nearest flagged cell of each request with
+    // flagged cells and every improvement strictly lowers a key, so the loop
+    // ends; the cap only turns an inconsistency into an error instead of a hang.
+    const uint64_t max_rounds = 256ull * max_extent + 4096;
+    // Batches of rounds are enqueued one ahead of the one being checked: while
+    // the host waits for batch k's frontier size, batch k+1 is already on the
+    // GPU. Once the front thins out the batches shrink, so fewer empty rounds
+    // run after the last one that took a cell.
     BatchProbe probe;
     int cur = 0;
     uint64_t round = 0;
     auto enqueue_batch = [&](int slot) {
         for (int b = 0; b < batch; ++b, ++round) {
-            SDFX_LAUNCH(k_frontier_compact, dim3(compact_blocks), dim3(256), 0, s, keys, g, changed, tile_changed,
-                        items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1], band, totals + (cur ^ 1),
-                        release, (int)SIGN);
-            SDFX_LAUNCH(k_frontier_propagate<SIGN>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
-                        d_tris, g, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], totals + cur,
-                        b == batch - 1 ? probe.word(slot) : nullptr, d_flags, evals ? evals + seed_slots : nullptr);
+            SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
+                        changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
+                        taken[cur ^ 1], taken[cur], (int)SIGN);
+            SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
+                        d_tris, rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
+                        totals + cur, b == batch - 1 ? probe.word(slot) : nullptr, d_flags,
+                        evals ? evals + seed_slots : nullptr);
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());
@@ -759,32 +1042,28 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     for (int slot = 0;; slot ^= 1) {
         if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
         enqueue_batch(slot ^ 1);
-        const uint32_t taken = probe.wait(slot);
-        if (taken == 0) break;  // the batch after it only runs empty rounds
-        batch = taken < 512 ? 2 : (taken < 8192 ? 4 : 8);
+        const uint32_t taken_cells = probe.wait(slot);
+        if (taken_cells == 0) break;  // the batch after it only runs empty rounds
+        batch = taken_cells < 512 ? 2 : (taken_cells < 8192 ? 4 : 8);
     }
 
     // 3. sign
-    uint32_t *bx = nullptr, *by = nullptr, *bz = nullptr;
+    uint32_t* bits[3] = {nullptr, nullptr, nullptr};
     if (SIGN == SDFX_SIGN_RAYCAST) {
-        uint64_t rows[3] = {(uint64_t)g.n[1] * g.n[2], (uint64_t)g.n[0] * g.n[2], (uint64_t)g.n[0] * g.n[1]};
-        uint32_t* bufs[3];
         for (int a = 0; a < 3; ++a) {
-            uint64_t nw = rows[a] * g.words[a];
-            bufs[a] = A.alloc<uint32_t>(nw);
-            SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(nw)), dim3(256), 0, s, bufs[a], nw, 0u);
+            bits[a] = A.alloc<uint32_t>(words_total[a]);
+            SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(words_total[a])), dim3(256), 0, s, bits[a], words_total[a],
+                        0u);
         }
-        bx = bufs[0];
-        by = bufs[1];
-        bz = bufs[2];
-        SDFX_LAUNCH(k_ray_scatter, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, d_tris, nt, g, bx, by, bz);
+        SDFX_LAUNCH(k_ray_scatter<MODE>, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, d_tris, nt, rv, tri_req,
+                    bits[0], bits[1], bits[2]);
         for (int a = 0; a < 3; ++a)
-            SDFX_LAUNCH(k_ray_suffix, dim3(blocks_for(rows[a], 256)), dim3(256), 0, s, bufs[a], rows[a],
-                               g.words[a]);
+            SDFX_LAUNCH(k_ray_suffix<MODE>, dim3(blocks_for(rows_total[a], 256)), dim3(256), 0, s, bits[a],
+                        rows_total[a], rv, nreq, a);
         SDFX_HIP(hipGetLastError());
     }
-    SDFX_LAUNCH(k_grid_finish<SIGN>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, g, bx, by, bz,
-                       d_out);
+    SDFX_LAUNCH(k_grid_finish<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, rv, tile_req,
+                bits[0], bits[1], bits[2]);
     SDFX_HIP(hipGetLastError());
     if (evals) {
         profile_count("grid.seed_distance_evals", sum_block_counters(evals, seed_slots, s));
@@ -792,26 +1071,37 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const GridDev& g, float*
     }
 }
 
+void grid_sdf_batch(const TriRec* d_tris, const std::vector<GridBatchItem>& items, sdfx_sign_method sign,
+                    DevFlags* d_flags, Arena& A) {
+    std::vector<GridBatchItem> jobs;
+    uint64_t nt = 0;
+    for (const GridBatchItem& it : items) {
+        const uint64_t ncells = it.gp.count[0] * it.gp.count[1] * it.gp.count[2];
+        if (!ncells) continue;
+        if (it.n_tris == 0) {
+            // no triangle: every cell keeps f32::MAX (grid.rs:137-143), no ray hits
+            SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(ncells)), dim3(256), 0, A.stream, (uint32_t*)it.d_out, ncells,
+                        fbits(FLT_MAX));
+            continue;
+        }
+        jobs.push_back(it);
+        nt = std::max<uint64_t>(nt, it.tri_off + it.n_tris);
+    }
+    SDFX_HIP(hipGetLastError());
+    if (jobs.empty()) return;
+    const bool batch = jobs.size() > 1;
+    if (sign == SDFX_SIGN_NORMAL) {
+        if (batch) grid_run<SDFX_SIGN_NORMAL | 2>(d_tris, nt, jobs, d_flags, A);
+        else grid_run<SDFX_SIGN_NORMAL>(d_tris, nt, jobs, d_flags, A);
+    } else {
+        if (batch) grid_run<SDFX_SIGN_RAYCAST | 2>(d_tris, nt, jobs, d_flags, A);
+        else grid_run<SDFX_SIGN_RAYCAST>(d_tris, nt, jobs, d_flags, A);
+    }
+}
+
 void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
               DevFlags* d_flags, Arena& A) {
-    const uint64_t ncells = gp.count[0] * gp.count[1] * gp.count[2];
-    if (!ncells) return;
-    GridDev g;
-    for (int k = 0; k < 3; ++k) {
-        g.first[k] = gp.first[k];
-        g.size[k] = gp.size[k];
-        g.n[k] = (uint32_t)gp.count[k];
-        g.words[k] = (uint32_t)((gp.count[k] + 31) / 32);
-    }
-    if (nt == 0) {
-        // no triangle: every cell keeps f32::MAX (grid.rs:137-143), no ray hits
-        SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(ncells)), dim3(256), 0, A.stream, (uint32_t*)d_out, ncells,
-                           fbits(FLT_MAX));
-        SDFX_HIP(hipGetLastError());
-        return;
-    }
-    if (sign == SDFX_SIGN_NORMAL) grid_run<SDFX_SIGN_NORMAL>(d_tris, nt, g, d_out, d_flags, A);
-    else grid_run<SDFX_SIGN_RAYCAST>(d_tris, nt, g, d_out, d_flags, A);
+    grid_sdf_batch(d_tris, {GridBatchItem{0, nt, gp, d_out, 0}}, sign, d_flags, A);
 }
 
 }  // namespace sdfx

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "sdfx_geo.hpp"
 #include "sdfx_runtime.hpp"
@@ -117,6 +118,20 @@ struct GridParams {
 // propagation to a fixpoint, grid-aligned raycast sign (Raycast only).
 void grid_sdf(const TriRec* d_tris, uint64_t n_tris, const GridParams& g, sdfx_sign_method sign, float* d_out,
               DevFlags* d_flags, Arena& arena);
+
+// One request of a batched generate_grid_sdf: its triangles are
+// d_tris[tri_off, tri_off + n_tris) of the batch's triangle array (in its
+// topology order), its flags d_flags[flag].
+struct GridBatchItem {
+    uint64_t tri_off, n_tris;
+    GridParams gp;
+    float* d_out;
+    uint32_t flag;
+};
+// All requests (one sign method) through the same launches; each result is
+// bit-identical to grid_sdf on that request alone.
+void grid_sdf_batch(const TriRec* d_tris, const std::vector<GridBatchItem>& items, sdfx_sign_method sign,
+                    DevFlags* d_flags, Arena& arena);
 
 // ---------------------------------------------------------------- utilities
 
