@@ -7,7 +7,12 @@ one full generate_grid_sdf call (topology decode, seeding, propagation to the
 fixpoint, raycast sign, output write) on vertex/index buffers already resident
 in HBM. One SDF sample = one output cell.
 
-    python bench.py [--steps K] [--warmup W] [--workload grid|rtree_bvh|bvh]
+    python bench.py [--steps K] [--warmup W] [--workload grid|rtree_bvh|bvh|grid_batch]
+
+grid_batch: one step = one sdfx_generate_grid_sdf_batch_device call over 256
+requests; the same requests are also run as a plain loop of
+sdfx_generate_grid_sdf_device calls and both forms are reported (samples/s,
+p50/p99 latency per request).
 
 Prints one JSON line. The CPU baseline is the oracle's restatement of the
 reference's threaded algorithm, timed on this host's cores.
@@ -26,6 +31,8 @@ WORKLOADS = {
     "grid": "generate_grid_sdf Raycast, jittered torus 65,536 tris, 128^3 cells",
     "rtree_bvh": "generate_sdf RtreeBvh, icosphere L3 1,280 tris, 10,000 uniform queries",
     "bvh": "generate_sdf Bvh(Raycast), noise sphere L7 327,680 tris, 4,194,304 queries",
+    "grid_batch": "batched generate_grid_sdf Raycast, 256 jittered icospheres/tori (1,280-327,680 tris, "
+                  "log-uniform), one grid each with axes log-uniform in [32, 128]",
 }
 
 
@@ -112,8 +119,46 @@ def main():
                 0, ctypes.c_void_p(dout.data_ptr()), sp))
 
         samples = ncells
+        mesh_bytes = 12 * len(v) + 12 * len(f)
         config = {"workload": WORKLOADS["grid"], "triangles": int(len(f)), "cells": int(ncells),
                   "sign": "Raycast"}
+    elif args.workload == "grid_batch":
+        meshes = S.config4()
+        reqs = (_lib.GridRequest * len(meshes))()
+        resident = []  # device buffers kept alive for the run
+        samples = tri_total = mesh_bytes = 0
+        for r, (mv, mf, mfirst, msize, mcount) in zip(reqs, meshes):
+            dvi = torch.from_numpy(mv).to(dev)
+            dfi = torch.from_numpy(mf.reshape(-1).astype(np.int32)).to(dev)
+            nc = mcount[0] * mcount[1] * mcount[2]
+            doi = torch.empty(nc, dtype=torch.float32, device=dev)
+            resident.append((dvi, dfi, doi))
+            r.vertices, r.vertex_count = dvi.data_ptr(), len(mv)
+            r.indices, r.index_count = dfi.data_ptr(), dfi.numel()
+            r.topology = 0
+            r.first_cell[:] = [float(x) for x in mfirst]
+            r.cell_size[:] = [float(x) for x in msize]
+            r.cell_count[:] = list(mcount)
+            r.out = doi.data_ptr()
+            samples += nc
+            tri_total += len(mf)
+            mesh_bytes += 12 * len(mv) + 12 * len(mf)
+        v, f = meshes[0][0], meshes[0][1]
+        F = ctypes.POINTER(ctypes.c_float)
+        U = ctypes.POINTER(ctypes.c_uint64)
+
+        def step():
+            _lib.check(L.sdfx_generate_grid_sdf_batch_device(reqs, len(meshes), 0, sp))
+
+        def plain_call(i):
+            r = reqs[i]
+            _lib.check(L.sdfx_generate_grid_sdf_device(
+                ctypes.c_void_p(r.vertices), r.vertex_count, ctypes.c_void_p(r.indices), r.index_count, 0,
+                ctypes.cast(r.first_cell, F), ctypes.cast(r.cell_size, F), ctypes.cast(r.cell_count, U), 0,
+                ctypes.c_void_p(r.out), sp))
+
+        config = {"workload": WORKLOADS["grid_batch"], "requests": len(meshes), "triangles": int(tri_total),
+                  "cells": int(samples), "sign": "Raycast"}
     else:
         if args.workload == "rtree_bvh":
             v, f, q = S.config1()
@@ -134,6 +179,7 @@ def main():
                 ctypes.byref(n), sp))
 
         samples = len(q)
+        mesh_bytes = 12 * len(v) + 12 * len(f)
         config = {"workload": WORKLOADS[args.workload], "triangles": int(len(f)), "queries": int(len(q))}
 
     for _ in range(args.warmup):
@@ -147,14 +193,46 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    elapsed = timed(args.steps)
+    step_ms = []
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+    elapsed = time.perf_counter() - t_all
     ms = elapsed * 1000.0 / args.steps
     value = samples * args.steps / elapsed
+    forms = None
+    if args.workload == "grid_batch":
+        # The same requests as a plain loop of single-grid calls: each call
+        # returns when its grid is done, so its time is that request's latency.
+        for i in range(len(meshes)):
+            plain_call(i)
+        lat, loops = [], max(1, min(3, args.steps))
+        torch.cuda.synchronize()
+        t_loop = time.perf_counter()
+        for _ in range(loops):
+            for i in range(len(meshes)):
+                t0 = time.perf_counter()
+                plain_call(i)
+                torch.cuda.synchronize()
+                lat.append((time.perf_counter() - t0) * 1e3)
+        loop_s = (time.perf_counter() - t_loop) / loops
+        pct = lambda a, p: float(np.percentile(np.asarray(a), p))
+        forms = {"batched": {"samples_per_s": value, "ms_per_batch": ms, "p50_ms": pct(step_ms, 50),
+                             "p99_ms": pct(step_ms, 99),
+                             "latency": "every request of a batch completes with the batch call"},
+                 "plain_loop": {"samples_per_s": samples / loop_s, "ms_per_pass": loop_s * 1e3,
+                                "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                                "latency": "per request (one sdfx_generate_grid_sdf_device call)"},
+                 "batched_over_plain": value / (samples / loop_s)}
 
     # Kernel timing: the same K steps again with HIP events around every launch
     # (on the launch stream), plus exact work counters.
     if args.no_kernel_timing:
-        print(json.dumps({"value": value, "ms_per_step": ms, "steps": args.steps}), flush=True)
+        print(json.dumps({"value": value, "ms_per_step": ms, "steps": args.steps, "forms": forms}), flush=True)
         return
     _lib.profile_enable(True)
     timed(args.steps)
@@ -172,13 +250,12 @@ def main():
     timed(args.steps)
     dom_ms, dom_launches = _lib.profile_read()[dom_name]
     _lib.profile_enable(False)
-    mesh_bytes = 12 * len(v) + 12 * len(f)
     # Algorithmic bytes per SDF sample (implementation-independent): a grid cell
     # writes 4 B, a point query reads 12 B and writes 4 B; the mesh (12 B per
     # vertex and per index triple) is read once per call. One call runs
     # `launches_per_step` launches of the dominant kernel, which are charged
     # the call's bytes in equal parts.
-    per_sample = 4.0 if args.workload == "grid" else 16.0
+    per_sample = 4.0 if args.workload.startswith("grid") else 16.0
     step_bytes = per_sample * samples + mesh_bytes
     launches_per_step = dom_launches / args.steps
     avg_launch_s = dom_ms / dom_launches / 1e3
@@ -188,7 +265,7 @@ def main():
                 "work_per_launch": f"{step_bytes / launches_per_step:.0f} B ({per_sample:.0f} B x {samples} samples "
                                    f"+ {mesh_bytes} B of mesh, over {launches_per_step:g} launches per call)"}
     valu = None
-    if args.workload == "grid":
+    if args.workload.startswith("grid"):
         # diagnostic beside the headline: FP32 VALU work of the distance evaluations
         # (80 operations per point-triangle distance, geo.rs:26-30, 70-138 as written)
         evals = None
@@ -221,6 +298,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {n: round(t / args.steps, 4) for n, t, _ in kernels[:8]},
     }
+    if forms:
+        result["forms"] = forms
     if valu:
         result["valu_distance_evals"] = valu
     if counters["lbvh.queries"]:
@@ -260,6 +339,23 @@ def cpu_baseline(workload, v, f, env):
                 "sample": f"full workload ({len(f)} tris, {samples} cells); the oracle's restatement of "
                           f"generate/grid.rs with the heap split over {threads} threads; median of {len(times)}",
                 "seconds": t, "gpu_vs_this_run": accuracy, "speedup": None}
+    if workload == "grid_batch":
+        # a bounded sample: the requests in order until about 20 s of CPU time
+        cells = 0
+        t_start = time.perf_counter()
+        used = 0
+        for mv, mf, mfirst, msize, mcount in env["meshes"]:
+            if time.perf_counter() - t_start > 20.0:
+                break
+            s, _, _ = O.generate_grid_sdf(mv, mf, O.TOPO_LIST, mfirst, msize, mcount, O.SIGN_RAYCAST, threads,
+                                          O.GRID_THREADED)
+            cells += mcount[0] * mcount[1] * mcount[2]
+            used += 1
+        t = time.perf_counter() - t_start
+        return {"value": cells / t, "unit": "samples/s", "cores": threads, "kind": "port",
+                "sample": f"the first {used} of the {len(env['meshes'])} requests ({cells} cells), one call each; "
+                          f"the oracle's restatement of generate/grid.rs with the heap split over {threads} threads",
+                "seconds": t, "speedup": None}
     if workload == "rtree_bvh":
         q = env["q"]
         t_start = time.perf_counter()

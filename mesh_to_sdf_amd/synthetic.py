@@ -142,6 +142,18 @@ def noise_sphere(level=7, amplitude=0.05, seed=SEED_BASE + 3):
     return vd.astype(np.float32), f
 
 
+_ICOSPHERES: dict = {}
+
+
+def jittered_icosphere(level: int, jitter=0.002, seed=SEED_BASE):
+    """Icosphere with each vertex's radius scaled by 1 + N(0, jitter)."""
+    if level not in _ICOSPHERES:
+        _ICOSPHERES[level] = icosphere(level)
+    v, f = _ICOSPHERES[level]
+    eps = _rng(seed).normal(0.0, jitter, size=len(v))
+    return (v.astype(np.float64) * (1.0 + eps)[:, None]).astype(np.float32), f.copy()
+
+
 def aabb(v: np.ndarray):
     return v.min(axis=0), v.max(axis=0)
 
@@ -212,3 +224,28 @@ def config3(n_queries=4_194_304, level=7, seed=SEED_BASE + 3):
     qs = surface_queries(v, f, n_s, 0.01 * diag, seed + 100)
     qu = uniform_queries(v, n_queries - n_s, seed + 200)
     return v, f, np.concatenate([qs, qu], axis=0)
+
+
+def config4(n_meshes=256, seed=SEED_BASE + 4, t_min=1_280, t_max=327_680, edge_min=32, edge_max=128):
+    """The batched workload: `n_meshes` watertight meshes, alternately jittered
+    icospheres and jittered tori, with triangle counts log-uniform in
+    [t_min, t_max] (the icosphere level, 20 * 4^level triangles, or the torus
+    segment counts 2N x N, 4N^2 triangles, nearest to the drawn target), each
+    with one grid whose axes are independently log-uniform in
+    [edge_min, edge_max] cells over its AABB expanded by 10%. Returns a list of
+    (vertices, faces, first_cell, cell_size, cell_count)."""
+    g = _rng(seed)
+    out = []
+    for i in range(n_meshes):
+        target = math.exp(g.uniform(math.log(t_min), math.log(t_max)))
+        mseed = seed + 1000 + i
+        if i % 2 == 0:
+            level = min(range(3, 8), key=lambda lv: abs(math.log(20 * 4 ** lv) - math.log(target)))
+            v, f = jittered_icosphere(level, 0.002, mseed)
+        else:
+            n = max(3, int(round(math.sqrt(target / 4.0))))
+            v, f = torus(1.0, float(g.uniform(0.25, 0.45)), 2 * n, n, 0.002, mseed)
+        counts = tuple(int(round(math.exp(g.uniform(math.log(edge_min), math.log(edge_max))))) for _ in range(3))
+        first, size, count = grid_for(v, counts)
+        out.append((v, f, first, size, count))
+    return out
