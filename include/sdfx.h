@@ -60,7 +60,9 @@ typedef enum sdfx_status {
     SDFX_ERR_DEVICE = -8,
     SDFX_ERR_OUT_OF_MEMORY = -9,
     /* grid.rs:311-315 clamp(0, n - 1) panics on a zero cell count */
-    SDFX_ERR_INVALID_GRID = -10
+    SDFX_ERR_INVALID_GRID = -10,
+    /* sdfx_server_*: the mesh or request does not fit the server's HBM budget */
+    SDFX_ERR_OVER_BUDGET = -11
 } sdfx_status;
 
 /* lib.rs:156-171 Topology::{TriangleList, TriangleStrip}; indices == NULL is `None` */
@@ -207,6 +209,81 @@ uint64_t sdfx_profile_counter(const char* name);
  * counters run only when counters != 0. Fewer event pairs disturb the launch
  * stream less, so one kernel's average duration is closest to a profiler's. */
 void sdfx_profile_select(const char* kernel_prefix, int counters);
+
+/* ---- Served form (no counterpart in the reference, which answers one call
+ * at a time on the caller's thread; for callers that answer a stream of
+ * requests against meshes kept on the device).
+ *
+ * A server keeps meshes resident (uploaded once: triangles in topology order
+ * for grid requests, an LBVH for point requests) and runs queued requests on
+ * `workers` threads, each with its own stream. A worker that comes free takes
+ * every queued request that fits the free part of the HBM budget, first come
+ * first served, and runs them as one batch: the grid requests of each sign
+ * method in one batched grid run, the point requests of each query mode in one
+ * multi-mesh LBVH launch. Requests submitted while batches run join the next
+ * batch (continuous batching); batches of different workers overlap on the
+ * device. Each answer is bit-identical to the request served alone.
+ *
+ * Budget: resident meshes count 256 bytes per triangle; a request in flight
+ * counts its scratch (about 28 bytes per grid cell plus 40 per triangle of its
+ * mesh, 24 bytes per query point). Query points and outputs are the caller's
+ * device buffers and are not counted; they must stay valid until the request
+ * has been waited for.
+ *
+ * Point requests answer query_sdf (lib.rs:463-485) on an acceleration of the
+ * given method; like it, they make no finiteness check of the points. Grid
+ * requests answer generate_grid_sdf (generate/grid.rs:265-273) on the mesh. */
+typedef struct sdfx_server sdfx_server;
+
+typedef struct sdfx_server_stats {
+    uint64_t requests_done;
+    uint64_t batches;             /* batches run (a batch is one worker's turn) */
+    uint64_t max_batch_requests;  /* most requests in one batch */
+    uint64_t resident_bytes;      /* budget held by resident meshes */
+    uint64_t inflight_bytes;      /* budget held by running requests */
+    uint64_t peak_inflight_bytes;
+    uint64_t meshes;
+    uint64_t queued;              /* requests waiting for a worker */
+} sdfx_server_stats;
+
+/* Starts a server on the current device with `workers` (1..8) worker threads. */
+sdfx_status sdfx_server_create(uint64_t hbm_budget_bytes, uint32_t workers, sdfx_server** server_out);
+
+/* Serves every queued request, then stops the workers and frees the meshes. */
+void sdfx_server_destroy(sdfx_server* server);
+
+/* Uploads a mesh (device pointers, copied: the caller may free them after the
+ * call) and builds its LBVH. SDFX_ERR_OVER_BUDGET when it cannot fit beside the
+ * resident meshes; otherwise waits until the requests in flight leave room.
+ * Topology and index errors as generate_grid_sdf reports them. */
+sdfx_status sdfx_server_add_mesh(sdfx_server* server, const float* d_vertices, uint64_t vertex_count,
+                                 const uint32_t* d_indices, uint64_t index_count, sdfx_topology topology,
+                                 uint32_t* mesh_id_out);
+
+/* Drops a mesh from the server (requests already queued for it still run). */
+sdfx_status sdfx_server_remove_mesh(sdfx_server* server, uint32_t mesh_id);
+
+/* Queues a point request: d_out[i] = the signed distance of d_query_points[i]
+ * (device pointers). The request's status (sdfx_generate_sdf's) comes from
+ * sdfx_server_wait. */
+sdfx_status sdfx_server_submit_points(sdfx_server* server, uint32_t mesh_id, const float* d_query_points,
+                                      uint64_t query_count, sdfx_acceleration_method method, sdfx_sign_method sign,
+                                      float* d_out, uint64_t* ticket_out);
+
+/* Queues a grid request on the mesh (d_out: device pointer, z fastest). */
+sdfx_status sdfx_server_submit_grid(sdfx_server* server, uint32_t mesh_id, const float first_cell[3],
+                                    const float cell_size[3], const uint64_t cell_count[3], sdfx_sign_method sign,
+                                    float* d_out, uint64_t* ticket_out);
+
+/* Blocks until the request is done; returns its status (its message through
+ * sdfx_last_error) and, when latency_ms is not NULL, the milliseconds from
+ * submission to completion. A ticket can be waited for once. */
+sdfx_status sdfx_server_wait(sdfx_server* server, uint64_t ticket, double* latency_ms);
+
+/* *done_out = 1 when the request is done (it must still be waited for). */
+sdfx_status sdfx_server_poll(sdfx_server* server, uint64_t ticket, int* done_out);
+
+sdfx_status sdfx_server_get_stats(sdfx_server* server, sdfx_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ ERR_INVALID_ARGUMENT = -7
 ERR_DEVICE = -8
 ERR_OUT_OF_MEMORY = -9
 ERR_INVALID_GRID = -10
+ERR_OVER_BUDGET = -11
 
 _f = ctypes.POINTER(ctypes.c_float)
 _u32 = ctypes.POINTER(ctypes.c_uint32)
@@ -41,6 +42,12 @@ class GridRequest(ctypes.Structure):
                 ("index_count", ctypes.c_uint64), ("topology", ctypes.c_int), ("first_cell", ctypes.c_float * 3),
                 ("cell_size", ctypes.c_float * 3), ("cell_count", ctypes.c_uint64 * 3), ("out", ctypes.c_void_p),
                 ("status", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    """sdfx_server_stats (include/sdfx.h)."""
+    _fields_ = [(n, ctypes.c_uint64) for n in ("requests_done", "batches", "max_batch_requests", "resident_bytes",
+                                                "inflight_bytes", "peak_inflight_bytes", "meshes", "queued")]
 
 
 # name -> (restype, argtypes); must match include/sdfx.h
@@ -59,6 +66,15 @@ SIGNATURES = {
     "sdfx_generate_grid_sdf_batch_device": (_i, [ctypes.POINTER(GridRequest), _u64v, _i, _vp]),
     "sdfx_check_mesh_triangle_list": (_i, [_f, _u64v, _u32, _u64v, ctypes.POINTER(_i), _u64, ctypes.c_char_p,
                                           _u64v]),
+    "sdfx_server_create": (_i, [_u64v, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "sdfx_server_destroy": (None, [_vp]),
+    "sdfx_server_add_mesh": (_i, [_vp, _vp, _u64v, _vp, _u64v, _i, _u32]),
+    "sdfx_server_remove_mesh": (_i, [_vp, ctypes.c_uint32]),
+    "sdfx_server_submit_points": (_i, [_vp, ctypes.c_uint32, _vp, _u64v, _i, _i, _vp, _u64]),
+    "sdfx_server_submit_grid": (_i, [_vp, ctypes.c_uint32, _f, _f, _u64, _i, _vp, _u64]),
+    "sdfx_server_wait": (_i, [_vp, _u64v, ctypes.POINTER(ctypes.c_double)]),
+    "sdfx_server_poll": (_i, [_vp, _u64v, ctypes.POINTER(_i)]),
+    "sdfx_server_get_stats": (_i, [_vp, ctypes.POINTER(ServerStats)]),
     "sdfx_profile_enable": (None, [_i]),
     "sdfx_profile_read": (_i, [_i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _u64]),
     "sdfx_profile_counter": (ctypes.c_uint64, [ctypes.c_char_p]),
@@ -90,6 +106,15 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C {CSRC}")
+        # One HIP runtime per process. torch ships its own libamdhip64 (soname
+        # libamdhip64.so.7) and loads it by another file name, so a process that
+        # loaded the system runtime first would end up with two, and the first
+        # one loses the GPU. Importing torch first makes libsdfx bind to torch's
+        # copy through the shared soname.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -275,11 +275,11 @@ sdfx_status grid_batch_core(sdfx_grid_request* reqs, uint64_t n, sdfx_sign_metho
         const sdfx_grid_request& r = reqs[i];
         gather_triangles(r.vertices, r.vertex_count, r.indices, r.topology, prep[i].nt, tris.get() + off, flags + i,
                          s);
-        if (!(prep[i].zero_axis && prep[i].nt > 0)) items.push_back(GridBatchItem{off, prep[i].nt, prep[i].g, r.out,
-                                                                                  (uint32_t)i});
+        if (!(prep[i].zero_axis && prep[i].nt > 0))
+            items.push_back(GridBatchItem{tris.get() + off, off, prep[i].nt, prep[i].g, r.out, (uint32_t)i});
         off += prep[i].nt;
     }
-    grid_sdf_batch(tris.get(), items, sign, flags, A);
+    grid_sdf_batch(items, sign, flags, A);
     std::vector<DevFlags> hf(n ? n : 1);
     SDFX_HIP(hipMemcpyAsync(hf.data(), flags, (n ? n : 1) * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
     SDFX_HIP(hipStreamSynchronize(s));

@@ -53,8 +53,9 @@ struct GridDev {
 // One request of a batch as the kernels see it.
 struct GridReq {
     GridDev g;
+    const TriRec* tris;     // the request's triangles, in its topology order
     uint64_t cell_off;      // first cell in the batch's key array (a multiple of COMPACT_TILE)
-    uint64_t tri_off;       // first triangle in the batch's triangle array
+    uint64_t tri_off;       // first triangle in the batch's triangle numbering
     uint64_t ntris;
     uint64_t row_off[3];    // first ray-parity word of each axis
     uint64_t row_first[3];  // first ray row of each axis in the batch's row numbering
@@ -252,8 +253,7 @@ struct SeedBox {
 // generate/grid.rs:400-426: padded box, snapped corners, widened by one cell
 // where the snapped corner's centre lies beyond the box.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ tris, uint64_t nt,
-                                                    ReqView<mode_batch<MODE>> reqs,
+__global__ void __launch_bounds__(256) k_seed_boxes(uint64_t nt, ReqView<mode_batch<MODE>> reqs,
                                                     const uint32_t* __restrict__ tri_req,
                                                     SeedBox* __restrict__ boxes, uint64_t* __restrict__ counts) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ t
         return;
     }
     const GridDev g = reqs[q].g;
-    const TriRec tr = tris[t];
+    const TriRec tr = reqs[q].tris[t - reqs[q].tri_off];
     Box b;
     tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
     uint32_t mn[3], mx[3];
@@ -292,8 +292,7 @@ __global__ void __launch_bounds__(256) k_seed_boxes(const TriRec* __restrict__ t
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ tris, uint64_t nt,
-                                                    ReqView<mode_batch<MODE>> reqs,
+__global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_batch<MODE>> reqs,
                                                     const uint32_t* __restrict__ tri_req,
                                                     const SeedBox* __restrict__ boxes,
                                                     const uint64_t* __restrict__ offsets,
@@ -324,7 +323,7 @@ __global__ void __launch_bounds__(256) k_seed_pairs(const TriRec* __restrict__ t
         uint64_t r = local % yz;
         uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
         uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
-        const TriRec tr = tris[t];
+        const TriRec tr = R.tris[t - R.tri_off];
         float d = cell_dist<SIGN>(R.g, x, y, z, tr);
         n_evals++;
         if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
@@ -513,14 +512,14 @@ struct PropAcc {
 // request q to its mesh-local triangle tri, kept if its key is below the
 // neighbour's current key.
 template <int MODE>
-__device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const ReqView<mode_batch<MODE>>& reqs,
+__device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
                                           uint64_t* keys, uint8_t* __restrict__ changed,
                                           uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
                                           DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
                                           uint32_t q, uint64_t cur, PropAcc& acc) {
     constexpr int SIGN = mode_sign<MODE>;
     const GridReq& R = reqs[q];
-    const TriRec tr = tris[R.tri_off + tri];
+    const TriRec tr = R.tris[tri];
     const float px = cell_coord(R.g, 0, x), py = cell_coord(R.g, 1, y), pz = cell_coord(R.g, 2, z);
     float dx, dy, dz, cx, cy, cz;
     closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
@@ -571,7 +570,6 @@ __device__ __forceinline__ void prop_eval(const TriRec* __restrict__ tris, const
 template <int MODE>
 __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
                                                             const uint32_t* __restrict__ ctr_cur,
-                                                            const TriRec* __restrict__ tris,
                                                             ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
                                                             uint64_t* keys, uint8_t* __restrict__ changed,
                                                             uint8_t* __restrict__ tile_changed,
@@ -662,14 +660,14 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
             qn -= 64u;
             const uint32_t e = qn + lane;
             const uint64_t c = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
-            prop_eval<MODE>(tris, reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][e], Q.y[wave][e],
+            prop_eval<MODE>(reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][e], Q.y[wave][e],
                             Q.z[wave][e], Q.tri[wave][e], mode_batch<MODE> ? Q.req[wave][e] : 0u, c, acc);
             wave_lds_sync();
         }
     }
     if (lane < qn) {
         const uint64_t c = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
-        prop_eval<MODE>(tris, reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][lane],
+        prop_eval<MODE>(reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][lane],
                         Q.y[wave][lane], Q.z[wave][lane], Q.tri[wave][lane],
                         mode_batch<MODE> ? Q.req[wave][lane] : 0u, c, acc);
     }
@@ -731,8 +729,7 @@ __device__ void scatter_axis(const TriRec& tr, const Box& b, const GridDev& g, u
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_ray_scatter(const TriRec* __restrict__ tris, uint64_t nt,
-                                                     ReqView<mode_batch<MODE>> reqs,
+__global__ void __launch_bounds__(256) k_ray_scatter(uint64_t nt, ReqView<mode_batch<MODE>> reqs,
                                                      const uint32_t* __restrict__ tri_req,
                                                      uint32_t* __restrict__ bx, uint32_t* __restrict__ by,
                                                      uint32_t* __restrict__ bz) {
@@ -743,7 +740,7 @@ __global__ void __launch_bounds__(256) k_ray_scatter(const TriRec* __restrict__ 
     const uint32_t q = mode_batch<MODE> ? tri_req[t] : 0u;
     if (q == UINT32_MAX) return;  // a triangle of no request in this run
     const GridReq& R = reqs[q];
-    const TriRec tr = tris[t];
+    const TriRec tr = R.tris[t - R.tri_off];
     Box b;
     tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
     if (axis == 0) scatter_axis<0>(tr, b, R.g, bx + R.row_off[0]);
@@ -901,8 +898,7 @@ static unsigned stride_blocks(uint64_t n) {
 }
 
 template <int MODE>
-static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFlags* d_flags,
-                     Arena& A) {
+static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFlags* d_flags, Arena& A) {
     constexpr int SIGN = mode_sign<MODE>;
     hipStream_t s = A.stream;
     const uint32_t nreq = (uint32_t)jobs.size();
@@ -923,6 +919,7 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBa
         const uint64_t nc = J.gp.count[0] * J.gp.count[1] * J.gp.count[2];
         R.cell_off = ncells;
         R.ncells = (uint32_t)nc;
+        R.tris = J.d_tris;
         R.tri_off = J.tri_off;
         R.ntris = J.n_tris;
         R.out = J.d_out;
@@ -970,8 +967,7 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBa
     SeedBox* boxes = A.alloc<SeedBox>(nt);
     uint64_t* counts = A.alloc<uint64_t>(nt);
     uint64_t* total = A.alloc<uint64_t>(1);
-    SDFX_LAUNCH(k_seed_boxes<MODE>, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_tris, nt, rv, tri_req, boxes,
-                counts);
+    SDFX_LAUNCH(k_seed_boxes<MODE>, dim3(blocks_for(nt, 256)), dim3(256), 0, s, nt, rv, tri_req, boxes, counts);
     exclusive_scan_u64(counts, counts, nt, total, A);
     // distance-evaluation counters, only while kernel timing is on
     // per-block slots: seeding (8192 blocks), then propagation (4096 blocks)
@@ -981,8 +977,8 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBa
         evals = A.alloc<unsigned long long>(seed_slots + prop_slots);
         fill_u32(reinterpret_cast<uint32_t*>(evals), 2 * (seed_slots + prop_slots), 0u, s);
     }
-    SDFX_LAUNCH(k_seed_pairs<MODE>, dim3(8192), dim3(256), 0, s, d_tris, nt, rv, tri_req, boxes, counts, total,
-                keys, d_flags, evals);
+    SDFX_LAUNCH(k_seed_pairs<MODE>, dim3(8192), dim3(256), 0, s, nt, rv, tri_req, boxes, counts, total, keys,
+                d_flags, evals);
     SDFX_HIP(hipGetLastError());
 
     // 2. frontier rounds
@@ -1030,7 +1026,7 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBa
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
                         taken[cur ^ 1], taken[cur], (int)SIGN);
             SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
-                        d_tris, rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
+                        rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
                         totals + cur, b == batch - 1 ? probe.word(slot) : nullptr, d_flags,
                         evals ? evals + seed_slots : nullptr);
             cur ^= 1;
@@ -1055,8 +1051,8 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBa
             SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(words_total[a])), dim3(256), 0, s, bits[a], words_total[a],
                         0u);
         }
-        SDFX_LAUNCH(k_ray_scatter<MODE>, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, d_tris, nt, rv, tri_req,
-                    bits[0], bits[1], bits[2]);
+        SDFX_LAUNCH(k_ray_scatter<MODE>, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, nt, rv, tri_req, bits[0],
+                    bits[1], bits[2]);
         for (int a = 0; a < 3; ++a)
             SDFX_LAUNCH(k_ray_suffix<MODE>, dim3(blocks_for(rows_total[a], 256)), dim3(256), 0, s, bits[a],
                         rows_total[a], rv, nreq, a);
@@ -1071,8 +1067,7 @@ static void grid_run(const TriRec* d_tris, uint64_t nt, const std::vector<GridBa
     }
 }
 
-void grid_sdf_batch(const TriRec* d_tris, const std::vector<GridBatchItem>& items, sdfx_sign_method sign,
-                    DevFlags* d_flags, Arena& A) {
+void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method sign, DevFlags* d_flags, Arena& A) {
     std::vector<GridBatchItem> jobs;
     uint64_t nt = 0;
     for (const GridBatchItem& it : items) {
@@ -1091,17 +1086,17 @@ void grid_sdf_batch(const TriRec* d_tris, const std::vector<GridBatchItem>& item
     if (jobs.empty()) return;
     const bool batch = jobs.size() > 1;
     if (sign == SDFX_SIGN_NORMAL) {
-        if (batch) grid_run<SDFX_SIGN_NORMAL | 2>(d_tris, nt, jobs, d_flags, A);
-        else grid_run<SDFX_SIGN_NORMAL>(d_tris, nt, jobs, d_flags, A);
+        if (batch) grid_run<SDFX_SIGN_NORMAL | 2>(nt, jobs, d_flags, A);
+        else grid_run<SDFX_SIGN_NORMAL>(nt, jobs, d_flags, A);
     } else {
-        if (batch) grid_run<SDFX_SIGN_RAYCAST | 2>(d_tris, nt, jobs, d_flags, A);
-        else grid_run<SDFX_SIGN_RAYCAST>(d_tris, nt, jobs, d_flags, A);
+        if (batch) grid_run<SDFX_SIGN_RAYCAST | 2>(nt, jobs, d_flags, A);
+        else grid_run<SDFX_SIGN_RAYCAST>(nt, jobs, d_flags, A);
     }
 }
 
 void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
               DevFlags* d_flags, Arena& A) {
-    grid_sdf_batch(d_tris, {GridBatchItem{0, nt, gp, d_out, 0}}, sign, d_flags, A);
+    grid_sdf_batch({GridBatchItem{d_tris, 0, nt, gp, d_out, 0}}, sign, d_flags, A);
 }
 
 }  // namespace sdfx

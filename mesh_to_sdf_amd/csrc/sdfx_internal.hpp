@@ -70,6 +70,20 @@ enum QueryMode : int {
 void lbvh_query(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, QueryMode mode, float* d_out,
                 DevFlags* d_flags, Arena& arena);
 
+// One request of a batched point query: queries d_q[0, nq) against the mesh
+// whose LBVH is `bvh`, answers to d_out, flags d_flags[flag].
+struct PointBatchItem {
+    const Lbvh* bvh;
+    const float* d_q;
+    uint64_t nq;
+    float* d_out;
+    uint32_t flag;
+};
+// lbvh_query for many (mesh, queries) requests of one mode in shared launches
+// (the queries of all requests sorted by request, then Morton code); each
+// answer is the one lbvh_query gives.
+void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, DevFlags* d_flags, Arena& arena);
+
 // generic/rtree_bvh.rs:217-276: closest - query and the 3-ray inside vote.
 void lbvh_closest_vectors(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, float* d_vectors,
                           uint8_t* d_inside, DevFlags* d_flags, Arena& arena);
@@ -119,10 +133,12 @@ struct GridParams {
 void grid_sdf(const TriRec* d_tris, uint64_t n_tris, const GridParams& g, sdfx_sign_method sign, float* d_out,
               DevFlags* d_flags, Arena& arena);
 
-// One request of a batched generate_grid_sdf: its triangles are
-// d_tris[tri_off, tri_off + n_tris) of the batch's triangle array (in its
-// topology order), its flags d_flags[flag].
+// One request of a batched generate_grid_sdf: its n_tris triangles (in its
+// topology order) are d_tris[0, n_tris), numbered [tri_off, tri_off + n_tris)
+// in the batch (the requests' ranges are disjoint and increasing); its flags are
+// d_flags[flag].
 struct GridBatchItem {
+    const TriRec* d_tris;
     uint64_t tri_off, n_tris;
     GridParams gp;
     float* d_out;
@@ -130,8 +146,8 @@ struct GridBatchItem {
 };
 // All requests (one sign method) through the same launches; each result is
 // bit-identical to grid_sdf on that request alone.
-void grid_sdf_batch(const TriRec* d_tris, const std::vector<GridBatchItem>& items, sdfx_sign_method sign,
-                    DevFlags* d_flags, Arena& arena);
+void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method sign, DevFlags* d_flags,
+                    Arena& arena);
 
 // ---------------------------------------------------------------- utilities
 

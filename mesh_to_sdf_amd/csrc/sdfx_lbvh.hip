@@ -640,6 +640,31 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
     return insides > 1;
 }
 
+// The answer of one query point in mode MODE (the per-query work of every
+// query kernel below).
+template <int MODE>
+__device__ __forceinline__ void query_point(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves,
+                                            int32_t root, uint32_t max_visits, const float* __restrict__ q,
+                                            uint64_t i, float* __restrict__ out, DevFlags* flags, TravStats& ts) {
+    const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+    bool overflow = false;
+    if (MODE == QM_NORMAL_FOLD) {
+        bool nan_seen = false;
+        out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
+        if (nan_seen) atomicOr(&flags->nan_distance, 1u);
+    } else {
+        Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
+        if (MODE == QM_NEAREST_SIGNED) {
+            out[i] = nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
+        } else {
+            float d = sqrtf(nb.d2);
+            if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
+            out[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? -d : d;
+        }
+    }
+    if (overflow) atomicOr(&flags->internal, 1u);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ leaves, int32_t root,
@@ -652,23 +677,7 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
     TravStats ts;
     if (active) {
         if (order) i = order[i];  // lanes of a wave take Morton-adjacent queries
-        const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
-        bool overflow = false;
-        if (MODE == QM_NORMAL_FOLD) {
-            bool nan_seen = false;
-            out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
-            if (nan_seen) atomicOr(&flags->nan_distance, 1u);
-        } else {
-            Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
-            if (MODE == QM_NEAREST_SIGNED) {
-                out[i] = nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
-            } else {
-                float d = sqrtf(nb.d2);
-                if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
-                out[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? -d : d;
-            }
-        }
-        if (overflow) atomicOr(&flags->internal, 1u);
+        query_point<MODE>(nodes, leaves, root, max_visits, q, i, out, flags, ts);
     }
     if (stats) {  // kernel argument: uniform, every lane of the block gets here
         const uint64_t nb = gridDim.x;
@@ -749,6 +758,126 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         profile_count("lbvh.vote_nodes", sum_block_counters(stats + 2ull * blocks, blocks, s));
         profile_count("lbvh.vote_leaves", sum_block_counters(stats + 3ull * blocks, blocks, s));
         profile_count("lbvh.queries", nq);
+    }
+}
+
+// ---------------------------------------------------------------- batched queries
+
+// One request of a batched point query as the kernels see it.
+struct PointReq {
+    const BvhNode* nodes;
+    const TriRec* leaves;
+    const float* q;
+    float* out;
+    uint64_t q_off;  // first query of the request in the batch numbering
+    int32_t root;
+    uint32_t max_visits;
+    uint32_t flag;
+    uint32_t pad;
+};
+
+// The request holding batch query g: the last one whose q_off is <= g.
+__device__ __forceinline__ uint32_t point_req_of(const PointReq* __restrict__ reqs, uint32_t nreq, uint64_t g) {
+    uint32_t lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].q_off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sort key of batch query g: its request in the top 8 bits, then a 24-bit
+// Morton code inside its mesh's root box, so a sort groups each request's
+// queries in spatial order.
+__global__ void __launch_bounds__(256) k_point_morton_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
+                                                            uint64_t total, uint32_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t r = point_req_of(reqs, nreq, g);
+    const PointReq& R = reqs[r];
+    const uint64_t i = g - R.q_off;
+    uint32_t code = 0;
+    if (R.root >= 0) {
+        const BvhNode n0 = R.nodes[0];
+        uint32_t c[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float lo = fminf(n0.lmin[k], n0.rmin[k]), hi = fmaxf(n0.lmax[k], n0.rmax[k]);
+            const float ext = hi - lo;
+            const float u = ext > 0.0f ? (R.q[3 * i + k] - lo) / ext : 0.0f;
+            const float sc = u * 256.0f;
+            c[k] = sc <= 0.0f ? 0u : (sc >= 255.0f ? 255u : (uint32_t)sc);  // NaN -> 0
+        }
+        code = (expand10(c[0]) << 2) | (expand10(c[1]) << 1) | expand10(c[2]);
+    }
+    keys[g] = (r << 24) | code;
+    vals[g] = (uint32_t)g;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
+                                                          uint64_t total, const uint32_t* __restrict__ order,
+                                                          DevFlags* flags) {
+    const uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t g = order ? order[i] : i;
+    const PointReq& R = reqs[point_req_of(reqs, nreq, g)];
+    TravStats ts;
+    query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag, ts);
+}
+
+void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, DevFlags* d_flags, Arena& A) {
+    hipStream_t s = A.stream;
+    constexpr size_t CHUNK = 256;  // requests per launch: the sort key holds the request in 8 bits
+    for (size_t c0 = 0; c0 < items.size(); c0 += CHUNK) {
+        const size_t c1 = std::min(items.size(), c0 + CHUNK);
+        std::vector<PointReq> h;
+        uint64_t total = 0;
+        for (size_t k = c0; k < c1; ++k) {
+            const PointBatchItem& it = items[k];
+            if (!it.nq || !it.bvh || !it.bvh->n) continue;
+            h.push_back(PointReq{it.bvh->nodes.get(), it.bvh->leaves.get(), it.d_q, it.d_out, total, it.bvh->root,
+                                 visit_cap(*it.bvh), it.flag, 0u});
+            total += it.nq;
+        }
+        if (h.empty()) continue;
+        if (total >= (1ull << 32)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "a point batch holds fewer than 2^32 queries");
+        PointReq* d_reqs = A.alloc<PointReq>(h.size());
+        SDFX_HIP(hipMemcpyAsync(d_reqs, h.data(), h.size() * sizeof(PointReq), hipMemcpyHostToDevice, s));
+        const uint32_t nreq = (uint32_t)h.size();
+        const uint32_t* order = nullptr;
+        if (total >= 4096) {
+            uint32_t* keys = A.alloc<uint32_t>(total);
+            uint32_t* vals = A.alloc<uint32_t>(total);
+            SDFX_LAUNCH(k_point_morton_batch, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_reqs, nreq, total, keys,
+                        vals);
+            int rbits = 0;  // the request field's width
+            while ((1u << rbits) < nreq) ++rbits;
+            radix_sort_pairs(keys, vals, total, 24 + rbits, A);
+            order = vals;
+        }
+        const unsigned blocks = order ? xcd_grid(blocks_for(total, 256)) : blocks_for(total, 256);
+        switch (mode) {
+            case QM_RAYCAST_VOTE:
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
+                            order, d_flags);
+                break;
+            case QM_NORMAL_FOLD:
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
+                            order, d_flags);
+                break;
+            case QM_NEAREST_SIGNED:
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
+                            total, order, d_flags);
+                break;
+            case QM_NEAREST_VOTE:
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
+                            order, d_flags);
+                break;
+        }
+        SDFX_HIP(hipGetLastError());
     }
 }
 

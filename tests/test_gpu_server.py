@@ -1,0 +1,142 @@
+"""The served form (sdfx_server_*): each answer is bit-identical to the same
+request answered alone through the plain entry points, requests queued while
+the device is busy are served in shared batches, the HBM budget gates meshes
+and requests, and a failing request leaves the others untouched."""
+import numpy as np
+import pytest
+
+from mesh_to_sdf_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def _meshes():
+    return [S.icosphere(2), S.torus(1.0, 0.35, 24, 12, 0.002, seed=5), S.noise_sphere(3, 0.05, seed=7),
+            S.jittered_icosphere(3, 0.002, seed=9)]
+
+
+def _server(budget=4 << 30, workers=2):
+    from mesh_to_sdf_amd.server import SdfServer
+    return SdfServer(budget, workers)
+
+
+def test_mixed_requests_equal_alone(sdf):
+    meshes = _meshes()
+    g = np.random.default_rng(11)
+    methods = list(sdf.ALL_SDF_ACCELERATION_METHODS)
+    with _server() as srv:
+        ids = [srv.add_mesh(v, sdf.Topology.TriangleList(f)) for v, f in meshes]
+        jobs = []
+        for k in range(36):
+            mi = int(g.integers(len(meshes)))
+            v, f = meshes[mi]
+            if k % 3 == 2:
+                counts = tuple(int(c) for c in g.integers(6, 48, size=3))
+                grid = sdf.Grid(*S.grid_for(v, counts, factor=float(g.uniform(0.9, 1.3))))
+                sign = int(g.integers(2))
+                jobs.append(("grid", mi, grid, sign, srv.submit_grid(ids[mi], grid, sign)))
+            else:
+                n = int(g.integers(1, 6000))
+                q = np.concatenate([S.uniform_queries(v, n - n // 2, seed=100 + k),
+                                    S.surface_queries(v, f, n // 2, 0.01, seed=200 + k)])
+                m = methods[k % len(methods)]
+                jobs.append(("points", mi, q, m, srv.submit_points(ids[mi], q, m)))
+        for kind, mi, a, b, tk in jobs:
+            got, ms = srv.wait(tk)
+            assert ms >= 0.0
+            v, f = meshes[mi]
+            if kind == "grid":
+                alone = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), a, b)
+            else:
+                alone = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), a, b)
+            assert np.array_equal(bits(got.cpu().numpy()), bits(alone)), (kind, mi, b)
+        st = srv.stats()
+        assert st["requests_done"] == len(jobs) and st["queued"] == 0 and st["inflight_bytes"] == 0
+
+
+def test_queued_requests_share_batches(sdf):
+    """With one worker busy on a large grid, the requests submitted meanwhile
+    wait in the queue and are then served together in one batch."""
+    v, f = S.torus(1.0, 0.35, 128, 64, 0.002, seed=3)
+    small = S.icosphere(2)
+    with _server(workers=1) as srv:
+        big = srv.add_mesh(v, sdf.Topology.TriangleList(f))
+        sm = srv.add_mesh(small[0], sdf.Topology.TriangleList(small[1]))
+        first = srv.submit_grid(big, sdf.Grid(*S.grid_for(v, (224, 224, 224))), 0)
+        rest = []
+        for k in range(12):
+            q = S.uniform_queries(small[0], 500 + 37 * k, seed=k)
+            rest.append((q, srv.submit_points(sm, q, sdf.AccelerationMethod.RtreeBvh)))
+        srv.wait(first)
+        for q, tk in rest:
+            got, _ = srv.wait(tk)
+            alone = sdf.generate_sdf(small[0], sdf.Topology.TriangleList(small[1]), q, sdf.AccelerationMethod.RtreeBvh)
+            assert np.array_equal(bits(got.cpu().numpy()), bits(alone))
+        st = srv.stats()
+        assert st["max_batch_requests"] >= 2, st
+        assert st["batches"] < 13, st
+
+
+def test_budget_gates_meshes_and_requests(sdf):
+    from mesh_to_sdf_amd import _lib
+    v, f = S.icosphere(3)  # 1,280 triangles: 256 B each resident
+    with _server(budget=2_000_000, workers=2) as srv:
+        a = srv.add_mesh(v, sdf.Topology.TriangleList(f))
+        big_v, big_f = S.icosphere(5)  # 20,480 triangles: 5.2 MB resident
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.add_mesh(big_v, sdf.Topology.TriangleList(big_f))
+        assert e.value.status == _lib.ERR_OVER_BUDGET
+        with pytest.raises(_lib.SdfxError) as e:  # 128^3 cells need about 59 MB of scratch
+            srv.submit_grid(a, sdf.Grid(*S.grid_for(v, (128, 128, 128))), 0)
+        assert e.value.status == _lib.ERR_OVER_BUDGET
+        # requests that fit one at a time but not all together still all run
+        grids = [sdf.Grid(*S.grid_for(v, (30, 30, 30 + k))) for k in range(6)]  # about 1 MB of scratch each
+        tks = [srv.submit_grid(a, gr, 0) for gr in grids]
+        for gr, tk in zip(grids, tks):
+            got, _ = srv.wait(tk)
+            alone = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), gr, 0)
+            assert np.array_equal(bits(got.cpu().numpy()), bits(alone))
+        st = srv.stats()
+        assert st["peak_inflight_bytes"] <= 2_000_000 - st["resident_bytes"], st
+
+
+def test_failures_are_isolated(sdf):
+    from mesh_to_sdf_amd import _lib
+    v, f = S.icosphere(2)
+    with _server() as srv:
+        a = srv.add_mesh(v, sdf.Topology.TriangleList(f))
+        empty = srv.add_mesh(np.zeros((0, 3), np.float32), sdf.Topology.TriangleList(np.zeros(0, np.uint32)))
+        bad_idx = f.reshape(-1).copy()
+        bad_idx[4] = len(v) + 1
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.add_mesh(v, sdf.Topology.TriangleList(bad_idx))
+        assert e.value.status == _lib.ERR_INDEX_OUT_OF_RANGE
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.add_mesh(v, sdf.Topology.TriangleList(bad_idx[:-1]))
+        assert e.value.status == _lib.ERR_INDEX_COUNT
+        q = S.uniform_queries(v, 300, seed=1)
+        t_ok = srv.submit_points(a, q, sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast))
+        t_zero = srv.submit_grid(a, sdf.Grid([0, 0, 0], [0.1, 0.1, 0.1], [4, 0, 4]), 0)
+        t_empty_rb = srv.submit_points(empty, q, sdf.AccelerationMethod.RtreeBvh)
+        t_empty_bvh = srv.submit_points(empty, q, sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast))
+        t_empty_grid = srv.submit_grid(empty, sdf.Grid([0, 0, 0], [0.1, 0.1, 0.1], [3, 4, 5]), 0)
+        got, _ = srv.wait(t_ok)
+        alone = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast))
+        assert np.array_equal(bits(got.cpu().numpy()), bits(alone))
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.wait(t_zero)
+        assert e.value.status == _lib.ERR_INVALID_GRID
+        assert srv.wait(t_empty_rb)[0] is None  # rtree_bvh.rs:96-98: empty Vec
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.wait(t_empty_bvh)
+        assert e.value.status == _lib.ERR_EMPTY_MESH
+        got, _ = srv.wait(t_empty_grid)  # no triangle: every cell keeps f32::MAX
+        assert np.all(got.cpu().numpy() == np.finfo(np.float32).max)
+        srv.remove_mesh(a)
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.submit_points(a, q, sdf.AccelerationMethod.RtreeBvh)
+        assert e.value.status == _lib.ERR_INVALID_ARGUMENT
