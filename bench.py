@@ -14,12 +14,20 @@ requests; the same requests are also run as a plain loop of
 sdfx_generate_grid_sdf_device calls and both forms are reported (samples/s,
 p50/p99 latency per request).
 
+serve: one step = the 256 requests submitted at once to a server
+(sdfx_server_*) holding the 64 meshes, until the last is answered (saturated
+throughput). Also reported: Poisson arrivals at each rate in SERVE_LAMBDAS
+(samples/s, p50/p99 latency from submission to completion), and the plain
+drop-in (sdfx_generate_sdf_device / sdfx_generate_grid_sdf_device, which
+rebuild their acceleration per call) running the same requests one by one.
+
 Prints one JSON line. The CPU baseline is the oracle's restatement of the
 reference's threaded algorithm, timed on this host's cores.
 """
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -33,7 +41,11 @@ WORKLOADS = {
     "bvh": "generate_sdf Bvh(Raycast), noise sphere L7 327,680 tris, 4,194,304 queries",
     "grid_batch": "batched generate_grid_sdf Raycast, 256 jittered icospheres/tori (1,280-327,680 tris, "
                   "log-uniform), one grid each with axes log-uniform in [32, 128]",
+    "serve": "served requests over 64 resident meshes (as in the batched workload): 256 requests, half point "
+             "queries (1k-1M points, log-uniform, half near-surface) with Bvh(Raycast), half grids (edge "
+             "log-uniform in [32, 128]) with Raycast",
 }
+SERVE_LAMBDAS = (50, 100, 200, 500, 1000, 2000)
 
 
 def parse():
@@ -159,6 +171,14 @@ def main():
 
         config = {"workload": WORKLOADS["grid_batch"], "requests": len(meshes), "triangles": int(tri_total),
                   "cells": int(samples), "sign": "Raycast"}
+    elif args.workload == "serve":
+        serve = ServeWorkload(torch, dev, L, sp)
+        step = serve.burst
+        samples, mesh_bytes = serve.samples, serve.mesh_bytes
+        v, f = serve.meshes[0][0], serve.meshes[0][1]
+        config = {"workload": WORKLOADS["serve"], "meshes": len(serve.meshes), "requests": len(serve.reqs),
+                  "point_queries": serve.point_samples, "grid_cells": serve.grid_samples,
+                  "workers": serve.workers, "hbm_budget_bytes": serve.budget}
     else:
         if args.workload == "rtree_bvh":
             v, f, q = S.config1()
@@ -228,6 +248,8 @@ def main():
                                 "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                                 "latency": "per request (one sdfx_generate_grid_sdf_device call)"},
                  "batched_over_plain": value / (samples / loop_s)}
+    if args.workload == "serve":
+        forms = serve.forms(value, ms, step_ms)
 
     # Kernel timing: the same K steps again with HIP events around every launch
     # (on the launch stream), plus exact work counters.
@@ -255,7 +277,9 @@ def main():
     # vertex and per index triple) is read once per call. One call runs
     # `launches_per_step` launches of the dominant kernel, which are charged
     # the call's bytes in equal parts.
-    per_sample = 4.0 if args.workload.startswith("grid") else 16.0
+    per_sample = (4.0 if args.workload.startswith("grid") else
+                  (4.0 * serve.grid_samples + 16.0 * serve.point_samples) / samples if args.workload == "serve"
+                  else 16.0)
     step_bytes = per_sample * samples + mesh_bytes
     launches_per_step = dom_launches / args.steps
     avg_launch_s = dom_ms / dom_launches / 1e3
@@ -311,6 +335,140 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+class ServeWorkload:
+    """64 resident meshes and a stream of 256 point / grid requests on a server."""
+
+    def __init__(self, torch, dev, L, sp, n_meshes=64, n_requests=256, workers=2, budget=32 << 30,
+                 seed=0x5DF00005):
+        import numpy as np
+        from mesh_to_sdf_amd import _lib, synthetic as S
+        self.torch, self.L, self.sp, self.np, self._lib = torch, L, sp, np, _lib
+        self.workers, self.budget = workers, budget
+        self.meshes = S.config4(n_meshes, seed=seed)
+        h = ctypes.c_void_p()
+        _lib.check(L.sdfx_server_create(budget, workers, ctypes.byref(h)))
+        self.h = h
+        self.dmesh, self.ids = [], []
+        self.mesh_bytes = 0
+        for mv, mf, *_ in self.meshes:
+            dvi = torch.from_numpy(mv).to(dev)
+            dfi = torch.from_numpy(mf.reshape(-1).astype(np.int32)).to(dev)
+            torch.cuda.synchronize()
+            mid = ctypes.c_uint32(0)
+            _lib.check(L.sdfx_server_add_mesh(h, dvi.data_ptr(), len(mv), dfi.data_ptr(), dfi.numel(), 0,
+                                              ctypes.byref(mid)))
+            self.dmesh.append((dvi, dfi))
+            self.ids.append(mid.value)
+        g = np.random.Generator(np.random.PCG64(seed + 1))
+        self.reqs = []
+        self.point_samples = self.grid_samples = 0
+        for k in range(n_requests):
+            mi = int(g.integers(n_meshes))
+            mv, mf = self.meshes[mi][0], self.meshes[mi][1]
+            if k % 2 == 0:
+                n = int(round(math.exp(g.uniform(math.log(1e3), math.log(1e6)))))
+                q = np.concatenate([S.surface_queries(mv, mf, n // 2, 0.01 * S.bbox_diagonal(mv), seed + 10 + k),
+                                    S.uniform_queries(mv, n - n // 2, seed + 20 + k)])
+                dq = torch.from_numpy(q).to(dev)
+                out = torch.empty(n, dtype=torch.float32, device=dev)
+                self.reqs.append(("points", mi, dq, out, n))
+                self.point_samples += n
+            else:
+                e = int(round(math.exp(g.uniform(math.log(32), math.log(128)))))
+                first, size, count = S.grid_for(mv, (e, e, e))
+                out = torch.empty(e * e * e, dtype=torch.float32, device=dev)
+                grid = ((ctypes.c_float * 3)(*first), (ctypes.c_float * 3)(*size), (ctypes.c_uint64 * 3)(*count))
+                self.reqs.append(("grid", mi, grid, out, e * e * e))
+                self.grid_samples += e * e * e
+        torch.cuda.synchronize()
+        self.samples = self.point_samples + self.grid_samples
+
+    def submit(self, r):
+        kind, mi, a, out, n = r
+        tk = ctypes.c_uint64(0)
+        if kind == "points":
+            st = self.L.sdfx_server_submit_points(self.h, self.ids[mi], a.data_ptr(), n, 1, 0, out.data_ptr(),
+                                                  ctypes.byref(tk))
+        else:
+            st = self.L.sdfx_server_submit_grid(self.h, self.ids[mi], a[0], a[1], a[2], 0, out.data_ptr(),
+                                                ctypes.byref(tk))
+        self._lib.check(st)
+        return tk.value
+
+    def wait(self, tk):
+        ms = ctypes.c_double(0.0)
+        self._lib.check(self.L.sdfx_server_wait(self.h, tk, ctypes.byref(ms)))
+        return ms.value
+
+    def burst(self):
+        tks = [self.submit(r) for r in self.reqs]
+        return [self.wait(t) for t in tks]
+
+    def poisson(self, lam, seed):
+        g = self.np.random.Generator(self.np.random.PCG64(seed))
+        gaps = g.exponential(1.0 / lam, size=len(self.reqs))
+        tks = []
+        t0 = time.perf_counter()
+        at = t0
+        for r, gap in zip(self.reqs, gaps):
+            at += gap
+            while time.perf_counter() < at:
+                pass
+            tks.append(self.submit(r))
+        lat = [self.wait(t) for t in tks]
+        return time.perf_counter() - t0, lat
+
+    def stats(self):
+        s = self._lib.ServerStats()
+        self._lib.check(self.L.sdfx_server_get_stats(self.h, ctypes.byref(s)))
+        return {n: int(getattr(s, n)) for n, _ in self._lib.ServerStats._fields_}
+
+    def plain(self):
+        """The plain drop-in, one request at a time (each point call builds its LBVH)."""
+        F = ctypes.POINTER(ctypes.c_float)
+        U = ctypes.POINTER(ctypes.c_uint64)
+        lat = []
+        t_all = time.perf_counter()
+        for kind, mi, a, out, n in self.reqs:
+            dvi, dfi = self.dmesh[mi]
+            t0 = time.perf_counter()
+            if kind == "points":
+                cnt = ctypes.c_uint64(0)
+                st = self.L.sdfx_generate_sdf_device(dvi.data_ptr(), dvi.shape[0], dfi.data_ptr(), dfi.numel(), 0,
+                                                     a.data_ptr(), n, 1, 0, out.data_ptr(), ctypes.byref(cnt), self.sp)
+            else:
+                st = self.L.sdfx_generate_grid_sdf_device(dvi.data_ptr(), dvi.shape[0], dfi.data_ptr(), dfi.numel(),
+                                                          0, ctypes.cast(a[0], F), ctypes.cast(a[1], F),
+                                                          ctypes.cast(a[2], U), 0, out.data_ptr(), self.sp)
+            self._lib.check(st)
+            lat.append((time.perf_counter() - t0) * 1e3)
+        return time.perf_counter() - t_all, lat
+
+    def forms(self, value, ms, step_ms):
+        np = self.np
+        pct = lambda a, p: float(np.percentile(np.asarray(a), p))
+        out = {"served_burst": {"samples_per_s": value, "ms_per_stream": ms,
+                                "note": "all requests submitted at once; saturated throughput"}}
+        sweep = []
+        for lam in SERVE_LAMBDAS:
+            b0 = self.stats()
+            span, lat = self.poisson(lam, 1000 + lam)
+            b1 = self.stats()
+            nb = b1["batches"] - b0["batches"]
+            sweep.append({"lambda_req_per_s": lam, "samples_per_s": self.samples / span, "seconds": span,
+                          "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "batches": nb,
+                          "requests_per_batch": len(self.reqs) / max(1, nb)})
+        out["served_poisson"] = sweep
+        self.plain()  # warm the plain path's allocations
+        span, lat = self.plain()
+        out["plain_sequential"] = {"samples_per_s": self.samples / span, "seconds": span, "p50_ms": pct(lat, 50),
+                                   "p99_ms": pct(lat, 99),
+                                   "note": "sdfx_generate_sdf_device / sdfx_generate_grid_sdf_device per request"}
+        out["served_over_plain"] = value / (self.samples / span)
+        out["server"] = self.stats()
+        return out
+
+
 def cpu_baseline(workload, v, f, env):
     """The oracle's restatement of the reference's CPU algorithm on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -355,6 +513,28 @@ def cpu_baseline(workload, v, f, env):
         return {"value": cells / t, "unit": "samples/s", "cores": threads, "kind": "port",
                 "sample": f"the first {used} of the {len(env['meshes'])} requests ({cells} cells), one call each; "
                           f"the oracle's restatement of generate/grid.rs with the heap split over {threads} threads",
+                "seconds": t, "speedup": None}
+    if workload == "serve":
+        # a bounded sample: the request stream in order until about 20 s of CPU
+        # time, each request through the oracle's restatement of its reference path
+        sv = env["serve"]
+        done = used = 0
+        t_start = time.perf_counter()
+        for kind, mi, a, _, n in sv.reqs:
+            if time.perf_counter() - t_start > 20.0:
+                break
+            mv, mf = sv.meshes[mi][0], sv.meshes[mi][1]
+            if kind == "points":
+                O.generate_sdf(mv, mf, O.TOPO_LIST, a.cpu().numpy(), O.ACCEL_BVH, O.SIGN_RAYCAST, threads)
+            else:
+                O.generate_grid_sdf(mv, mf, O.TOPO_LIST, np.array(a[0][:], np.float32), np.array(a[1][:], np.float32),
+                                    tuple(int(c) for c in a[2]), O.SIGN_RAYCAST, threads, O.GRID_THREADED)
+            done += n
+            used += 1
+        t = time.perf_counter() - t_start
+        return {"value": done / t, "unit": "samples/s", "cores": threads, "kind": "port",
+                "sample": f"the first {used} of the {len(sv.reqs)} requests ({done} samples) through the oracle's "
+                          f"restatement of generic/bvh.rs (points) and generate/grid.rs (grids), {threads} threads",
                 "seconds": t, "speedup": None}
     if workload == "rtree_bvh":
         q = env["q"]
