@@ -187,11 +187,14 @@ sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex
                                           uint64_t face_index_count, int* kind_out, uint64_t* where_out,
                                           char* message, uint64_t message_capacity);
 
-/* ---- Tuning (no counterpart in the reference). Process-wide knobs; returns 0
- * on success, -1 for an unknown key.
+/* ---- Tuning (no counterpart in the reference). Process-wide knobs, none of
+ * which changes a result; returns 0 on success, -1 for an unknown key or a
+ * value out of range.
  *   "exhaustive_max_pairs": queries x triangles at or below which Bvh, Rtree
  *   and RtreeBvh evaluate every triangle instead of building the LBVH (results
- *   are identical either way); <= 0 always builds the LBVH. */
+ *   are identical either way); <= 0 always builds the LBVH.
+ *   "grid_prop_blocks": blocks per grid propagation launch, 8..65536 (rounded
+ *   down to a multiple of 8; default 4096). */
 int sdfx_set_tuning(const char* key, int64_t value);
 int64_t sdfx_get_tuning(const char* key);
 

@@ -391,22 +391,30 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
                                                           const uint32_t* __restrict__ taken_prev,
                                                           uint32_t* __restrict__ taken_cur, int sign) {
     __shared__ uint32_t wave_tot[4], wave_min[4];
-    __shared__ uint32_t block_base;
+    __shared__ uint32_t block_base, band_min, prev_taken;
     if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
+        if (!tile_changed[tile_id]) continue;  // block-uniform: no flagged cell in the tile
         const uint32_t q = mode_batch<MODE> ? tile_req[tile_id] : 0u;
         const GridReq& R = reqs[q];
         const uint32_t local0 = (uint32_t)((uint64_t)tile_id * COMPACT_TILE - R.cell_off);
         const uint32_t c0 = local0 + threadIdx.x * COMPACT_PER_LANE;  // first cell of this lane, request-local
         const uint64_t g0 = R.cell_off + c0;                           // the same cell in the batch
-        // the tile flag, this lane's 16 cell flags and the band words are read together
+        // this lane's 16 cell flags, and the request's band words (read by one
+        // wave, shared through LDS), are read together
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        const uint8_t tflag = tile_changed[tile_id];
         if (c0 < R.ncells) v = *reinterpret_cast<const uint4*>(changed + g0);  // whole tiles are allocated
-        const float mf = bitsf(req_min<MODE>(min_cur, q));
-        const uint32_t prev = req_sum<MODE>(taken_prev, q);
-        if (!tflag) continue;  // block-uniform: no flagged cell in the tile
+        if (wave == 0) {
+            const uint32_t m = req_min<MODE>(min_cur, q), p = req_sum<MODE>(taken_prev, q);
+            if (lane == 0) {
+                band_min = m;
+                prev_taken = p;
+            }
+        }
+        __syncthreads();
+        const float mf = bitsf(band_min);
+        const uint32_t prev = prev_taken;
         float limit = mf + R.band;
         // a NaN band (or no flagged cell) or a thin previous front: every flagged cell
         if (!(limit >= mf) || prev < R.release) limit = INFINITY;
@@ -477,7 +485,7 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
             for (int k = 0; k < COMPACT_PER_LANE; ++k) w[k >> 2] |= ((deferred >> k) & 1u) << (8 * (k & 3));
             *reinterpret_cast<uint4*>(changed + g0) = make_uint4(w[0], w[1], w[2], w[3]);
         }
-        __syncthreads();  // wave_tot / wave_min / block_base are reused by the next tile
+        __syncthreads();  // wave_tot / wave_min / block_base / band_min are reused by the next tile
     }
 }
 
@@ -1006,7 +1014,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     SDFX_LAUNCH(k_mark_seeded<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, tile_req, changed,
                 tile_changed, minw[0]);
     SDFX_HIP(hipGetLastError());
-    const unsigned prop_blocks = 4096;
+    const unsigned prop_blocks = (unsigned)std::max<int64_t>(XCDS, tuning().grid_prop_blocks) / XCDS * XCDS;
     const unsigned compact_blocks = std::min<uint32_t>(ntiles, 4096u);
     int batch = 8;
     // Every round takes at least the nearest flagged cell of each request with

@@ -113,10 +113,13 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits, Aren
 
 // ---------------------------------------------------------------- tuning
 
-// Process-wide knobs (sdfx_set_tuning). exhaustive_max_pairs: queries x
-// triangles at or below which the accelerated modes run exhaustively.
+// Process-wide knobs (sdfx_set_tuning); none changes a result.
+// exhaustive_max_pairs: queries x triangles at or below which the accelerated
+// modes run exhaustively. grid_prop_blocks: blocks of each grid propagation
+// launch (rounded down to a multiple of the 8 XCDs).
 struct Tuning {
     int64_t exhaustive_max_pairs;
+    int64_t grid_prop_blocks;
 };
 Tuning& tuning();
 

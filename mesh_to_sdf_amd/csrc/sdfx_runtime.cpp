@@ -156,7 +156,7 @@ namespace sdfx {
 Tuning& tuning() {
     // 2^26 pairs: about 10k queries against 6k triangles, where the exhaustive
     // kernel finishes well before an LBVH build and traversal would.
-    static Tuning t{int64_t(1) << 26};
+    static Tuning t{int64_t(1) << 26, 4096};
     return t;
 }
 
@@ -267,18 +267,27 @@ void sdfx_profile_select(const char* kernel_prefix, int counters) {
     P.counters_on = counters != 0;
 }
 
+namespace {
+int64_t* tuning_slot(const char* key) {
+    if (!key) return nullptr;
+    const std::string k(key);
+    if (k == "exhaustive_max_pairs") return &sdfx::tuning().exhaustive_max_pairs;
+    if (k == "grid_prop_blocks") return &sdfx::tuning().grid_prop_blocks;
+    return nullptr;
+}
+}  // namespace
+
 int sdfx_set_tuning(const char* key, int64_t value) {
-    if (!key) return -1;
-    if (std::string(key) == "exhaustive_max_pairs") {
-        sdfx::tuning().exhaustive_max_pairs = value;
-        return 0;
-    }
-    return -1;
+    int64_t* slot = tuning_slot(key);
+    if (!slot) return -1;
+    if (std::string(key) == "grid_prop_blocks" && (value < 8 || value > 65536)) return -1;
+    *slot = value;
+    return 0;
 }
 
 int64_t sdfx_get_tuning(const char* key) {
-    if (key && std::string(key) == "exhaustive_max_pairs") return sdfx::tuning().exhaustive_max_pairs;
-    return -1;
+    int64_t* slot = tuning_slot(key);
+    return slot ? *slot : -1;
 }
 
 uint64_t sdfx_profile_counter(const char* name) {
