@@ -77,16 +77,16 @@ def calibrate_copy():
     print(json.dumps({"copies": 3, "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n}))
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/traffic.json: mean FETCH_SIZE / WRITE_SIZE KiB per dispatch, each
-    scaled by the reading the same counter gives for a copy of known size), or
-    None when there is no entry for it."""
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` in `workload` from the committed counter
+    summary (profiles/traffic.json, keyed "workload/kernel": mean FETCH_SIZE /
+    WRITE_SIZE KiB per dispatch, each scaled by the reading the same counter
+    gives for a copy of known size), or None when there is no entry for it."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        t = json.load(fh).get(kernel)
+        t = json.load(fh).get(f"{workload}/{kernel}")
     if not t:
         return None
     return 1024.0 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] + t["write_kib_per_dispatch"] * t["write_scale"])
@@ -261,7 +261,8 @@ def main():
     prof = _lib.profile_read()
     counters = {k: _lib.profile_counter(k) for k in (
         "grid.seed_distance_evals", "grid.propagate_distance_evals", "lbvh.nearest_nodes", "lbvh.nearest_leaves",
-        "lbvh.vote_nodes", "lbvh.vote_leaves", "lbvh.queries")}
+        "lbvh.vote_nodes", "lbvh.vote_leaves", "lbvh.queries", "lbvh.nearest_wave_slots",
+        "lbvh.vote_wave_slots")}
     _lib.profile_enable(False)
     kernels = sorted(((name, t, n) for name, (t, n) in prof.items()), key=lambda x: -x[1])
     dom_name = kernels[0][0]
@@ -285,7 +286,7 @@ def main():
     avg_launch_s = dom_ms / dom_launches / 1e3
     achieved = step_bytes / launches_per_step / avg_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                "traffic": measured_traffic(dom_name), "kernel": dom_name, "avg_launch_ms": dom_ms / dom_launches,
+                "traffic": measured_traffic(args.workload, dom_name), "kernel": dom_name, "avg_launch_ms": dom_ms / dom_launches,
                 "work_per_launch": f"{step_bytes / launches_per_step:.0f} B ({per_sample:.0f} B x {samples} samples "
                                    f"+ {mesh_bytes} B of mesh, over {launches_per_step:g} launches per call)"}
     valu = None
@@ -330,6 +331,11 @@ def main():
         nq_total = counters["lbvh.queries"]
         result["traversal_per_query"] = {k.split(".", 1)[1]: counters[k] / nq_total for k in (
             "lbvh.nearest_nodes", "lbvh.nearest_leaves", "lbvh.vote_nodes", "lbvh.vote_leaves")}
+        # SIMD efficiency: node visits / (64 x the slowest lane's visits, per wave)
+        for k in ("nearest", "vote"):
+            if counters[f"lbvh.{k}_wave_slots"]:
+                result["traversal_per_query"][f"{k}_simd_efficiency"] = (
+                    counters[f"lbvh.{k}_nodes"] / counters[f"lbvh.{k}_wave_slots"])
     if cpu and cpu.get("speedup") is None:
         cpu["speedup"] = value / cpu["value"] if cpu["value"] else None
     print(json.dumps(result), flush=True)

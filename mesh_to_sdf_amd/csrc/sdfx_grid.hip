@@ -62,6 +62,7 @@ struct GridReq {
     float* out;
     uint32_t ncells;
     float band;        // width of the distance band a round takes
+    float far;         // a thin round with the band minimum beyond this makes every later round take all
     uint32_t release;  // after a round that took fewer cells, a round takes every flagged cell
     uint32_t flag;     // index of the request's DevFlags
 };
@@ -146,6 +147,11 @@ constexpr int COMPACT_PER_LANE = 16;
 constexpr int COMPACT_TILE = 256 * COMPACT_PER_LANE;
 // Width of the distance band a round takes, in units of the largest |cell size|.
 constexpr float GRID_BAND_CELLS = 3.0f;
+// A thin round (below) whose band minimum lies beyond GRID_FAR_CELLS largest
+// |cell size|s makes every later round of its request take every flagged cell:
+// in the far field the front carries mostly final values and the band only
+// adds rounds.
+constexpr float GRID_FAR_CELLS = 8.0f;
 // A round after one that took fewer than max(GRID_RELEASE_MIN, cells /
 // GRID_RELEASE_DIV) cells takes every flagged cell: once the front is thin,
 // waiting for the band only adds rounds.
@@ -389,9 +395,10 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
                                                           const uint32_t* __restrict__ min_cur,
                                                           uint32_t* __restrict__ min_next,
                                                           const uint32_t* __restrict__ taken_prev,
-                                                          uint32_t* __restrict__ taken_cur, int sign) {
+                                                          uint32_t* __restrict__ taken_cur,
+                                                          uint32_t* __restrict__ released, int sign) {
     __shared__ uint32_t wave_tot[4], wave_min[4];
-    __shared__ uint32_t block_base, band_min, prev_taken;
+    __shared__ uint32_t block_base, band_min, prev_taken, far_released;
     if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
@@ -410,14 +417,18 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
             if (lane == 0) {
                 band_min = m;
                 prev_taken = p;
+                far_released = released[q];
             }
         }
         __syncthreads();
         const float mf = bitsf(band_min);
-        const uint32_t prev = prev_taken;
+        const bool thin = prev_taken < R.release;
+        // Every block taking a tile of q this round decides the same; the flag
+        // is read by the later rounds' compactions.
+        if (thin && mf > R.far && !far_released && threadIdx.x == 0) released[q] = 1u;
         float limit = mf + R.band;
-        // a NaN band (or no flagged cell) or a thin previous front: every flagged cell
-        if (!(limit >= mf) || prev < R.release) limit = INFINITY;
+        // a NaN band (or no flagged cell), a thin previous front, or the far field: every flagged cell
+        if (!(limit >= mf) || thin || far_released) limit = INFINITY;
         const uint32_t lbits = fbits(limit);  // non-negative floats order as their bits
         uint32_t flagged = 0;  // bit k: cell c0 + k is flagged
         {
@@ -932,8 +943,10 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         R.ntris = J.n_tris;
         R.out = J.d_out;
         R.flag = J.flag;
-        R.band = GRID_BAND_CELLS * std::fmax(std::fmax(std::fabs(R.g.size[0]), std::fabs(R.g.size[1])),
-                                             std::fabs(R.g.size[2]));
+        const float max_size = std::fmax(std::fmax(std::fabs(R.g.size[0]), std::fabs(R.g.size[1])),
+                                         std::fabs(R.g.size[2]));
+        R.band = GRID_BAND_CELLS * max_size;
+        R.far = GRID_FAR_CELLS * max_size;
         R.release = (uint32_t)std::max<uint64_t>(GRID_RELEASE_MIN, nc / GRID_RELEASE_DIV);
         const uint64_t rows[3] = {J.gp.count[1] * J.gp.count[2], J.gp.count[0] * J.gp.count[2],
                                   J.gp.count[0] * J.gp.count[1]};
@@ -994,11 +1007,12 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // minima and its taken-cell counts; then the two frontier totals
     const uint64_t cls_words = (uint64_t)NCLASS * WORD_STRIDE;
     const uint64_t req_words = (uint64_t)nreq * req_spread<MODE> * WORD_STRIDE;
-    uint32_t* words = A.alloc<uint32_t>(2 * cls_words + 4 * req_words + 2);
+    uint32_t* words = A.alloc<uint32_t>(2 * cls_words + 4 * req_words + 2 + nreq);
     uint32_t* ctr[2] = {words, words + cls_words};
     uint32_t* minw[2] = {words + 2 * cls_words, words + 2 * cls_words + req_words};
     uint32_t* taken[2] = {words + 2 * cls_words + 2 * req_words, words + 2 * cls_words + 3 * req_words};
     uint32_t* totals = words + 2 * cls_words + 4 * req_words;
+    uint32_t* released = totals + 2;  // per request: the far-field release happened
     const uint32_t region = (uint32_t)((ntiles + NCLASS - 1) / NCLASS * COMPACT_TILE);  // items per class
     uint8_t* changed = A.alloc<uint8_t>(ncells);  // whole tiles: 16-byte vector reads stay inside
     uint8_t* tile_changed = A.alloc<uint8_t>(ntiles);
@@ -1010,7 +1024,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // (2^24 per word: the sum over a request's words, at least 2^28, exceeds
     // every release threshold and stays below 2^32)
     SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(req_words)), dim3(256), 0, s, taken[1], req_words, 1u << 24);
-    fill_u32(totals, 2, 0u, s);
+    fill_u32(totals, 2 + nreq, 0u, s);
     SDFX_LAUNCH(k_mark_seeded<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, tile_req, changed,
                 tile_changed, minw[0]);
     SDFX_HIP(hipGetLastError());
@@ -1032,7 +1046,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
-                        taken[cur ^ 1], taken[cur], (int)SIGN);
+                        taken[cur ^ 1], taken[cur], released, (int)SIGN);
             SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
                         rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
                         totals + cur, b == batch - 1 ? probe.word(slot) : nullptr, d_flags,

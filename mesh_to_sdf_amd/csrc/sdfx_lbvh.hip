@@ -685,6 +685,15 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
         block_count(stats + nb, ts.leaves);
         block_count(stats + 2 * nb, ts.vote_nodes);
         block_count(stats + 3 * nb, ts.vote_leaves);
+        // lane slots the wave spends: 64 x its slowest lane's node visits (a
+        // wave steps until every lane is done)
+        uint32_t mx = ts.nodes, mv = ts.vote_nodes;
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+            mv = max(mv, (uint32_t)__shfl_xor((int)mv, off, 64));
+        }
+        block_count(stats + 4 * nb, (threadIdx.x & 63u) == 0 ? 64u * mx : 0u);
+        block_count(stats + 5 * nb, (threadIdx.x & 63u) == 0 ? 64u * mv : 0u);
     }
 }
 
@@ -730,8 +739,8 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
     unsigned long long* stats = nullptr;
     if (profile_enabled() && mode != QM_NORMAL_FOLD) {
-        stats = A.alloc<unsigned long long>(4ull * blocks);
-        fill_u32(reinterpret_cast<uint32_t*>(stats), 8ull * blocks, 0u, s);
+        stats = A.alloc<unsigned long long>(6ull * blocks);
+        fill_u32(reinterpret_cast<uint32_t*>(stats), 12ull * blocks, 0u, s);
     }
     switch (mode) {
         case QM_RAYCAST_VOTE:
@@ -757,6 +766,8 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         profile_count("lbvh.nearest_leaves", sum_block_counters(stats + blocks, blocks, s));
         profile_count("lbvh.vote_nodes", sum_block_counters(stats + 2ull * blocks, blocks, s));
         profile_count("lbvh.vote_leaves", sum_block_counters(stats + 3ull * blocks, blocks, s));
+        profile_count("lbvh.nearest_wave_slots", sum_block_counters(stats + 4ull * blocks, blocks, s));
+        profile_count("lbvh.vote_wave_slots", sum_block_counters(stats + 5ull * blocks, blocks, s));
         profile_count("lbvh.queries", nq);
     }
 }

@@ -1060,10 +1060,13 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
     if (seed_out)
         for (uint64_t i = 0; i < ncells; ++i) seed_out[i] = key_value(keys[i], sign);
     for (uint64_t i = 0; i < ncells; ++i) changed[i] = keys[i] != KEY_EMPTY;
-    const float band = ORC_GRID_BAND_CELLS * fmaxf(fmaxf(fabsf(g->size.x), fabsf(g->size.y)), fabsf(g->size.z));
+    const float max_size = fmaxf(fmaxf(fabsf(g->size.x), fabsf(g->size.y)), fabsf(g->size.z));
+    const float band = ORC_GRID_BAND_CELLS * max_size;
+    const float far = ORC_GRID_FAR_CELLS * max_size;
     const uint64_t release = ncells / ORC_GRID_RELEASE_DIV > ORC_GRID_RELEASE_MIN ? ncells / ORC_GRID_RELEASE_DIV
                                                                                   : ORC_GRID_RELEASE_MIN;
     uint64_t prev_taken = UINT64_MAX; /* round 0 keeps the band */
+    int released = 0;                 /* a thin round beyond `far` happened: every later round takes all */
     int64_t rounds = 0;
     uint64_t n_evals = 0;
     for (;;) {
@@ -1075,7 +1078,9 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
         float mf;
         memcpy(&mf, &m, 4);
         float limit = mf + band;
-        if (!(limit >= mf) || prev_taken < release) limit = INFINITY; /* a NaN band takes every flagged cell */
+        if (prev_taken < release && mf > far) released = 1;
+        /* a NaN band takes every flagged cell */
+        if (!(limit >= mf) || prev_taken < release || released) limit = INFINITY;
         uint64_t nf = 0;
         for (uint64_t i = 0; i < ncells; ++i) {
             if (!changed[i]) continue;

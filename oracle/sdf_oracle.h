@@ -103,12 +103,16 @@ typedef struct {
  * triangles before its near ones arrive. A round after one that took fewer
  * than max(ORC_GRID_RELEASE_MIN, cells / ORC_GRID_RELEASE_DIV) cells takes
  * every flagged cell instead (T_r = infinity): once the front is thin, waiting
- * only adds rounds. It is deterministic and is the schedule the device
- * pipeline runs, so it pins the device kernels bit-exactly; its distance from
- * the reference's own schedules is measured separately. */
+ * only adds rounds. When such a thin round happens with m_r beyond
+ * ORC_GRID_FAR_CELLS * max|cell_size| (FP32), every later round takes every
+ * flagged cell: in the far field the front carries mostly final values, so the
+ * band there only adds rounds. It is deterministic and is the schedule the
+ * device pipeline runs, so it pins the device kernels bit-exactly; its
+ * distance from the reference's own schedules is measured separately. */
 #define ORC_GRID_BAND_CELLS 3.0f
 #define ORC_GRID_RELEASE_MIN 1024u
 #define ORC_GRID_RELEASE_DIV 128u
+#define ORC_GRID_FAR_CELLS 8.0f
 int orc_generate_grid_sdf(const float* vertices, uint64_t n_vertices, const uint32_t* indices, uint64_t n_indices,
                           int topology, const float* first_cell, const float* cell_size,
                           const uint64_t* cell_count, int sign, float* out, int n_heaps, int schedule,
