@@ -132,13 +132,32 @@ __device__ __forceinline__ float cell_dist(const GridDev& g, uint32_t x, uint32_
     return SIGN == SDFX_SIGN_RAYCAST ? tri_dist(px, py, pz, t) : tri_signed_dist(px, py, pz, t);
 }
 
-__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
 __global__ void k_fill_u32g(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// One launch that sets a grid run's state: every key empty, then the u32 words
+// in runs of (count, value) (the round counters and band words, the ray-parity
+// bits).
+struct WordRuns {
+    uint64_t end[6];  // run k covers words [end[k-1], end[k])
+    uint32_t value[6];
+};
+__global__ void __launch_bounds__(256) k_grid_init(uint64_t* __restrict__ keys, uint64_t ncells,
+                                                   uint32_t* __restrict__ words, WordRuns runs) {
+    const uint64_t nwords = runs.end[5];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells + nwords;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < ncells) {
+            keys[i] = KEY_EMPTY;
+        } else {
+            const uint64_t w = i - ncells;
+            int k = 0;
+            while (w >= runs.end[k]) ++k;  // w < end[5]: k stays below 6
+            words[w] = runs.value[k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- round bookkeeping
@@ -771,10 +790,24 @@ __global__ void __launch_bounds__(256) k_ray_scatter(uint64_t nt, ReqView<mode_b
 // last counted cell is at or beyond i). Rows of all requests along one axis are
 // numbered consecutively; a row finds its request by binary search.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_ray_suffix(uint32_t* __restrict__ bits, uint64_t rows,
-                                                    ReqView<mode_batch<MODE>> reqs, uint32_t nreq, int axis) {
+__global__ void __launch_bounds__(256) k_ray_suffix(uint32_t* __restrict__ bx, uint32_t* __restrict__ by,
+                                                    uint32_t* __restrict__ bz, uint64_t rows_x, uint64_t rows_y,
+                                                    uint64_t rows_z, ReqView<mode_batch<MODE>> reqs, uint32_t nreq) {
     uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    // the rows of the three axes, one after the other
+    int axis = 0;
+    uint32_t* bits = bx;
+    if (r >= rows_x) {
+        r -= rows_x;
+        axis = 1;
+        bits = by;
+        if (r >= rows_y) {
+            r -= rows_y;
+            axis = 2;
+            bits = bz;
+            if (r >= rows_z) return;
+        }
+    }
     uint32_t lo = 0, hi = nreq - 1;  // the last request whose first row is <= r
     while (lo < hi) {
         uint32_t mid = (lo + hi + 1) >> 1;
@@ -981,8 +1014,30 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     }
     ReqView<mode_batch<MODE>> rv{d_reqs, reqs[0]};
 
+    // state of the run: keys; per parity of the round NCLASS item counters,
+    // then per request its band minima and its taken-cell counts; the two
+    // frontier totals and the per-request far-field flags; the ray-parity bits
     uint64_t* keys = A.alloc<uint64_t>(ncells);
-    SDFX_LAUNCH(k_fill_u64, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, KEY_EMPTY);
+    const uint64_t cls_words = (uint64_t)NCLASS * WORD_STRIDE;
+    const uint64_t req_words = (uint64_t)nreq * req_spread<MODE> * WORD_STRIDE;
+    const uint64_t bit_words = SIGN == SDFX_SIGN_RAYCAST ? words_total[0] + words_total[1] + words_total[2] : 0;
+    WordRuns runs;
+    runs.end[0] = 2 * cls_words;                   // item counters: 0
+    runs.end[1] = runs.end[0] + 2 * req_words;     // band minima: none yet
+    runs.end[2] = runs.end[1] + req_words;         // taken counts of even rounds: 0
+    runs.end[3] = runs.end[2] + req_words;         // taken counts read by round 0: large, so the band applies
+    runs.end[4] = runs.end[3] + 2 + nreq;          // totals, far-field flags: 0
+    runs.end[5] = runs.end[4] + bit_words;         // ray-parity bits: 0
+    runs.value[0] = 0u;
+    runs.value[1] = UINT32_MAX;
+    runs.value[2] = 0u;
+    // 2^24 per word: the sum over a request's words, at least 2^28, exceeds
+    // every release threshold and stays below 2^32
+    runs.value[3] = 1u << 24;
+    runs.value[4] = 0u;
+    runs.value[5] = 0u;
+    uint32_t* words = A.alloc<uint32_t>(runs.end[5]);
+    SDFX_LAUNCH(k_grid_init, dim3(stride_blocks(ncells + runs.end[5])), dim3(256), 0, s, keys, ncells, words, runs);
 
     // 1. seeding
     SeedBox* boxes = A.alloc<SeedBox>(nt);
@@ -1003,11 +1058,6 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     SDFX_HIP(hipGetLastError());
 
     // 2. frontier rounds
-    // per parity of the round: NCLASS item counters, then per request its band
-    // minima and its taken-cell counts; then the two frontier totals
-    const uint64_t cls_words = (uint64_t)NCLASS * WORD_STRIDE;
-    const uint64_t req_words = (uint64_t)nreq * req_spread<MODE> * WORD_STRIDE;
-    uint32_t* words = A.alloc<uint32_t>(2 * cls_words + 4 * req_words + 2 + nreq);
     uint32_t* ctr[2] = {words, words + cls_words};
     uint32_t* minw[2] = {words + 2 * cls_words, words + 2 * cls_words + req_words};
     uint32_t* taken[2] = {words + 2 * cls_words + 2 * req_words, words + 2 * cls_words + 3 * req_words};
@@ -1017,14 +1067,6 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     uint8_t* changed = A.alloc<uint8_t>(ncells);  // whole tiles: 16-byte vector reads stay inside
     uint8_t* tile_changed = A.alloc<uint8_t>(ntiles);
     FrontierItem* items = A.alloc<FrontierItem>((uint64_t)region * NCLASS);
-    fill_u32(ctr[0], 2 * cls_words, 0u, s);
-    fill_u32(minw[0], 2 * req_words, UINT32_MAX, s);
-    fill_u32(taken[0], req_words, 0u, s);
-    // round 0 reads taken[1] as its previous round: large, so the band applies
-    // (2^24 per word: the sum over a request's words, at least 2^28, exceeds
-    // every release threshold and stays below 2^32)
-    SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(req_words)), dim3(256), 0, s, taken[1], req_words, 1u << 24);
-    fill_u32(totals, 2 + nreq, 0u, s);
     SDFX_LAUNCH(k_mark_seeded<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, tile_req, changed,
                 tile_changed, minw[0]);
     SDFX_HIP(hipGetLastError());
@@ -1068,16 +1110,14 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // 3. sign
     uint32_t* bits[3] = {nullptr, nullptr, nullptr};
     if (SIGN == SDFX_SIGN_RAYCAST) {
-        for (int a = 0; a < 3; ++a) {
-            bits[a] = A.alloc<uint32_t>(words_total[a]);
-            SDFX_LAUNCH(k_fill_u32g, dim3(stride_blocks(words_total[a])), dim3(256), 0, s, bits[a], words_total[a],
-                        0u);
-        }
+        bits[0] = words + runs.end[4];  // cleared by k_grid_init
+        bits[1] = bits[0] + words_total[0];
+        bits[2] = bits[1] + words_total[1];
         SDFX_LAUNCH(k_ray_scatter<MODE>, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, nt, rv, tri_req, bits[0],
                     bits[1], bits[2]);
-        for (int a = 0; a < 3; ++a)
-            SDFX_LAUNCH(k_ray_suffix<MODE>, dim3(blocks_for(rows_total[a], 256)), dim3(256), 0, s, bits[a],
-                        rows_total[a], rv, nreq, a);
+        SDFX_LAUNCH(k_ray_suffix<MODE>, dim3(blocks_for(rows_total[0] + rows_total[1] + rows_total[2], 256)),
+                    dim3(256), 0, s, bits[0], bits[1], bits[2], rows_total[0], rows_total[1], rows_total[2], rv,
+                    nreq);
         SDFX_HIP(hipGetLastError());
     }
     SDFX_LAUNCH(k_grid_finish<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, rv, tile_req,
