@@ -33,6 +33,7 @@
 // keys hold mesh-local triangle indices, so every result is bit-identical to
 // the same request run alone.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -604,7 +605,9 @@ __device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
 // unneeded evaluation or atomic. Improved distances are folded into each
 // request's min_next words; min_cur and taken_next (read by this round's
 // compaction, written by the next one) are reset. The frontier size goes to
-// *total_out and, when host_total is not null, to that pinned host word.
+// *total_out and, when host_probe is not null, to host_probe[0] in pinned host
+// memory, followed (after a system-scope fence) by host_seq in host_probe[1]:
+// the host reads the size once it sees the sequence number.
 template <int MODE>
 __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
                                                             const uint32_t* __restrict__ ctr_cur,
@@ -615,7 +618,8 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
                                                             uint32_t* __restrict__ min_cur,
                                                             uint32_t* __restrict__ taken_next,
                                                             uint32_t* __restrict__ total_out,
-                                                            uint32_t* __restrict__ host_total, DevFlags* flags,
+                                                            uint32_t* __restrict__ host_probe, uint32_t host_seq,
+                                                            DevFlags* flags,
                                                             unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
     // prefix[c]: items in the classes before c
@@ -633,9 +637,10 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
         if (threadIdx.x == 0) prefix[0] = 0;
         if (blockIdx.x == 0 && threadIdx.x == 63) {
             *total_out = x;
-            if (host_total) {
-                *host_total = x;
+            if (host_probe) {
+                __hip_atomic_store(&host_probe[0], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 __threadfence_system();
+                __hip_atomic_store(&host_probe[1], host_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
     }
@@ -909,37 +914,61 @@ static void* pinned_stage(size_t bytes) {
     return st.p;
 }
 
-// Two pinned words per thread and their events: the last propagation of a
-// batch of rounds stores its frontier size straight into one of the words (no
-// copy kernel), and the host reads it after that batch's event while the next
-// batch of rounds runs.
+// Two pinned probes per thread: the last propagation of a batch of rounds
+// stores its frontier size and then the batch's sequence number straight into
+// one of them (no copy kernel, no event in the stream: an event record costs a
+// few microseconds of idle GPU between batches), and the host polls for the
+// sequence number while the next batch of rounds runs.
 struct BatchProbe {
+    static constexpr uint32_t STRIDE = 16;  // u32 per probe: one 64-byte line each
     struct Slots {
         uint32_t* words = nullptr;    // host address
         uint32_t* d_words = nullptr;  // the same words as the device addresses them
-        hipEvent_t ev[2] = {nullptr, nullptr};
+        uint32_t seq = 0;             // last sequence number handed out on this thread
+        uint32_t want[2] = {0, 0};
         ~Slots() {
             if (words) (void)hipHostFree(words);
-            for (hipEvent_t e : ev)
-                if (e) (void)hipEventDestroy(e);
         }
     };
     Slots& slots() {
         static thread_local Slots sl;
         if (!sl.words) {
-            SDFX_HIP(hipHostMalloc((void**)&sl.words, 2 * sizeof(uint32_t),
+            SDFX_HIP(hipHostMalloc((void**)&sl.words, 2 * STRIDE * sizeof(uint32_t),
                                    hipHostMallocMapped | hipHostMallocCoherent));
+            std::memset(sl.words, 0, 2 * STRIDE * sizeof(uint32_t));
             SDFX_HIP(hipHostGetDevicePointer((void**)&sl.d_words, sl.words, 0));
-            for (auto& e : sl.ev) SDFX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
         return sl;
     }
-    uint32_t* word(int slot) { return &slots().d_words[slot]; }
-    void record(int slot, hipStream_t s) { SDFX_HIP(hipEventRecord(slots().ev[slot], s)); }
-    uint32_t wait(int slot) {
+    uint32_t* probe(int slot) { return slots().d_words + slot * STRIDE; }
+    // the sequence number the next batch in `slot` reports
+    uint32_t arm(int slot) {
         Slots& sl = slots();
-        SDFX_HIP(hipEventSynchronize(sl.ev[slot]));
-        return sl.words[slot];
+        sl.want[slot] = ++sl.seq ? sl.seq : ++sl.seq;  // never 0, the words' initial value
+        return sl.want[slot];
+    }
+    // The frontier size reported by the batch armed in `slot`. The batch's
+    // launches are already in stream s: if s goes idle without the report, or
+    // a minute passes, the run failed and an error is raised instead of waiting.
+    uint32_t wait(int slot, hipStream_t s) {
+        Slots& sl = slots();
+        const uint32_t* w = sl.words + slot * STRIDE;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint64_t spin = 1;; ++spin) {
+            if (__atomic_load_n(&w[1], __ATOMIC_ACQUIRE) == sl.want[slot]) break;
+            if ((spin & 4095u) == 0) {
+                const hipError_t q = hipStreamQuery(s);
+                if (q == hipSuccess) {
+                    if (__atomic_load_n(&w[1], __ATOMIC_ACQUIRE) == sl.want[slot]) break;
+                    throw Error(SDFX_ERR_DEVICE, "grid round batch ended without reporting its frontier");
+                }
+                if (q != hipErrorNotReady) SDFX_HIP(q);
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60))
+                    throw Error(SDFX_ERR_DEVICE, "grid round batch did not report its frontier within 60 s");
+            }
+            __builtin_ia32_pause();
+        }
+        return __atomic_load_n(&w[0], __ATOMIC_ACQUIRE);
     }
 };
 
@@ -1085,24 +1114,24 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     int cur = 0;
     uint64_t round = 0;
     auto enqueue_batch = [&](int slot) {
+        const uint32_t seq = probe.arm(slot);
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
                         taken[cur ^ 1], taken[cur], released, (int)SIGN);
             SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
                         rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
-                        totals + cur, b == batch - 1 ? probe.word(slot) : nullptr, d_flags,
+                        totals + cur, b == batch - 1 ? probe.probe(slot) : nullptr, seq, d_flags,
                         evals ? evals + seed_slots : nullptr);
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());
-        probe.record(slot, s);  // after the batch's last frontier size is stored
     };
     enqueue_batch(0);
     for (int slot = 0;; slot ^= 1) {
         if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
         enqueue_batch(slot ^ 1);
-        const uint32_t taken_cells = probe.wait(slot);
+        const uint32_t taken_cells = probe.wait(slot, s);
         if (taken_cells == 0) break;  // the batch after it only runs empty rounds
         batch = taken_cells < 512 ? 2 : (taken_cells < 8192 ? 4 : 8);
     }
