@@ -41,6 +41,7 @@ WORKLOADS = {
     "bvh": "generate_sdf Bvh(Raycast), noise sphere L7 327,680 tris, 4,194,304 queries",
     "grid_batch": "batched generate_grid_sdf Raycast, 256 jittered icospheres/tori (1,280-327,680 tris, "
                   "log-uniform), one grid each with axes log-uniform in [32, 128]",
+    "mesh_check": "check_mesh_triangle_list on the noise sphere L7 (327,680 tris, 163,842 vertices)",
     "serve": "served requests over 64 resident meshes (as in the batched workload): 256 requests, half point "
              "queries (1k-1M points, log-uniform, half near-surface) with Bvh(Raycast), half grids (edge "
              "log-uniform in [32, 128]) with Raycast",
@@ -171,6 +172,23 @@ def main():
 
         config = {"workload": WORKLOADS["grid_batch"], "requests": len(meshes), "triangles": int(tri_total),
                   "cells": int(samples), "sign": "Raycast"}
+    elif args.workload == "mesh_check":
+        v, f, _ = S.config3(n_queries=16)
+        fv = np.ascontiguousarray(f.reshape(-1), np.uint32)
+        kind, where = ctypes.c_int(0), ctypes.c_uint64(0)
+        msg = ctypes.create_string_buffer(512)
+        Fp = ctypes.POINTER(ctypes.c_float)
+
+        def step():
+            st = L.sdfx_check_mesh_triangle_list(v.ctypes.data_as(Fp), len(v), fv.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_uint32)), fv.size, ctypes.byref(kind), ctypes.byref(where), msg, 512)
+            if st != _lib.OK:
+                raise RuntimeError(f"bench mesh is not watertight: {msg.value.decode()}")
+
+        samples = len(f)
+        mesh_bytes = 12 * len(v) + 12 * len(f)
+        config = {"workload": WORKLOADS["mesh_check"], "triangles": int(len(f)), "vertices": int(len(v)),
+                  "unit_of_work": "triangles checked (host arrays in, verdict out)"}
     elif args.workload == "serve":
         serve = ServeWorkload(torch, dev, L, sp)
         step = serve.burst
@@ -278,7 +296,7 @@ def main():
     # vertex and per index triple) is read once per call. One call runs
     # `launches_per_step` launches of the dominant kernel, which are charged
     # the call's bytes in equal parts.
-    per_sample = (4.0 if args.workload.startswith("grid") else
+    per_sample = (0.0 if args.workload == "mesh_check" else 4.0 if args.workload.startswith("grid") else
                   (4.0 * serve.grid_samples + 16.0 * serve.point_samples) / samples if args.workload == "serve"
                   else 16.0)
     step_bytes = per_sample * samples + mesh_bytes
@@ -305,10 +323,11 @@ def main():
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, v, f, locals())
 
+    checking = args.workload == "mesh_check"
     result = {
-        "metric": "SDF samples/sec (1 GPU)",
+        "metric": "triangles checked/sec (1 GPU)" if checking else "SDF samples/sec (1 GPU)",
         "value": value,
-        "unit": "samples/s",
+        "unit": "triangles/s" if checking else "samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -520,6 +539,18 @@ def cpu_baseline(workload, v, f, env):
                 "sample": f"the first {used} of the {len(env['meshes'])} requests ({cells} cells), one call each; "
                           f"the oracle's restatement of generate/grid.rs with the heap split over {threads} threads",
                 "seconds": t, "speedup": None}
+    if workload == "mesh_check":
+        # the oracle restates the reference's O(n^2) scans as written: a smaller
+        # mesh of the same family keeps it bounded
+        sv, sf = S.icosphere(4)
+        t0 = time.perf_counter()
+        k, _ = O.check_mesh_triangle_list_at(sv, sf)
+        t = time.perf_counter() - t0
+        assert k == 0
+        return {"value": len(sf) / t, "unit": "triangles/s", "cores": 1, "kind": "port",
+                "sample": f"icosphere L4 ({len(sf)} triangles): the reference's scans are O(n^2) "
+                          "(generate/mesh_check.rs:34-77), so the full mesh is out of reach; the rate falls "
+                          "linearly with the mesh size", "seconds": t, "speedup": None}
     if workload == "serve":
         # a bounded sample: the request stream in order until about 20 s of CPU
         # time, each request through the oracle's restatement of its reference path

@@ -122,3 +122,14 @@ def test_large_meshes(sdf):
     opened = {(int(x), int(y)) for x, y in zip(f.reshape(-1, 3)[1234], np.roll(f.reshape(-1, 3)[1234], -1))}
     edge = tuple(int(t) for t in msg.split("edge ")[1].split(" does")[0].split(" -> "))
     assert (edge[1], edge[0]) in opened  # an edge whose partner was in the removed face
+
+
+def test_bench_meshes_watertight(sdf):
+    """Every mesh the workloads run (the three single-mesh configurations, the
+    256 of the batched workload and the 64 of the served workload) is
+    watertight, the precondition of the Raycast sign (generate/mesh_check.rs)."""
+    meshes = [S.config1()[:2], S.config2()[:2], S.config3(n_queries=16)[:2]]
+    meshes += [(m[0], m[1]) for m in S.config4()]
+    meshes += [(m[0], m[1]) for m in S.config4(64, seed=0x5DF00005)]  # the served workload's pool
+    for i, (v, f) in enumerate(meshes):
+        assert sdf.check_mesh_triangle_list(v, f) is None, f"mesh {i}"
