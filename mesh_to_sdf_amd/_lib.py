@@ -121,7 +121,26 @@ def lib() -> ctypes.CDLL:
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        apply_tuning_file()
     return _lib
+
+
+TUNING_PATH = os.path.join(HERE, "tuning.json")
+
+
+def apply_tuning_file(path: str = None) -> dict:
+    """Applies the knobs of a tuning file written by mesh_to_sdf_amd.autotune
+    (default: SDFX_TUNING, else mesh_to_sdf_amd/tuning.json; SDFX_TUNING=none
+    skips it). The knobs change launch shapes only, never a result."""
+    import json
+    path = path or os.environ.get("SDFX_TUNING", TUNING_PATH)
+    if path == "none" or not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        knobs = json.load(fh).get("knobs", {})
+    for key, value in knobs.items():
+        set_tuning(key, int(value))
+    return knobs
 
 
 def profile_enable(on: bool = True) -> None:
