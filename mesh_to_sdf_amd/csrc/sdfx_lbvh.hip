@@ -38,9 +38,13 @@ __device__ __forceinline__ void tri_centroid(const TriRec& t, float* c) {
     for (int k = 0; k < 3; ++k) c[k] = 0.5f * (b.lo[k] + b.hi[k]);
 }
 
+// The six bound words (min x, y, z as ordered ints, then max) sit 64 u32
+// (256 bytes) apart: device-scope atomics on one cache line serialise.
+constexpr uint32_t BOUND_STRIDE = 64;
+
 __global__ void k_init_bounds(uint32_t* ob) {
-    if (threadIdx.x < 3) ob[threadIdx.x] = 0xFFFFFFFFu;
-    else if (threadIdx.x < 6) ob[threadIdx.x] = 0u;
+    if (threadIdx.x < 3) ob[threadIdx.x * BOUND_STRIDE] = 0xFFFFFFFFu;
+    else if (threadIdx.x < 6) ob[threadIdx.x * BOUND_STRIDE] = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_centroid_bounds(const TriRec* __restrict__ tris, uint64_t n,
@@ -63,11 +67,26 @@ __global__ void __launch_bounds__(256) k_centroid_bounds(const TriRec* __restric
             mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off, 64));
         }
     }
+    // block-wide, then one check-before-atomic per block and word
+    __shared__ uint32_t wmn[4][3], wmx[4][3];
+    const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&ob[k], mn[k]);
-            atomicMax(&ob[3 + k], mx[k]);
+            wmn[wave][k] = mn[k];
+            wmx[wave][k] = mx[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x % 3;
+        uint32_t* w = &ob[threadIdx.x * BOUND_STRIDE];
+        if (threadIdx.x < 3) {
+            const uint32_t m = min(min(wmn[0][k], wmn[1][k]), min(wmn[2][k], wmn[3][k]));
+            if (m < __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(w, m);
+        } else {
+            const uint32_t m = max(max(wmx[0][k], wmx[1][k]), max(wmx[2][k], wmx[3][k]));
+            if (m > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(w, m);
         }
     }
 }
@@ -91,7 +110,7 @@ __global__ void __launch_bounds__(256) k_morton(const TriRec* __restrict__ tris,
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float lo = ord2f(ob[k]), hi = ord2f(ob[3 + k]);
+        float lo = ord2f(ob[k * BOUND_STRIDE]), hi = ord2f(ob[(3 + k) * BOUND_STRIDE]);
         float ext = hi - lo;
         float u = ext > 0.0f ? (c[k] - lo) / ext : 0.0f;
         float s = u * 1024.0f;
@@ -256,18 +275,34 @@ __global__ void __launch_bounds__(256) k_leaves(const TriRec* __restrict__ tris,
     leaf_box[i] = b;
 }
 
-// A box another lane wrote before its release fence: read around the L1.
-__device__ __forceinline__ Box load_box_coherent(const Box* b) {
-    const volatile float* f = (const volatile float*)b;
+// Boxes handed from one thread to another (possibly on another XCD) are
+// stored and read with agent-scope atomics, which go to the device's coherence
+// point instead of an XCD's own L2. Only the counter that says "both children
+// are done" then has to be ordered after them, which a workgroup-scope fence
+// (a wait for the stores to complete) does without writing back or
+// invalidating the L2 as a device-scope fence would on every tree level.
+__device__ __forceinline__ void store_box_agent(Box* b, const Box& v) {
+    float* f = reinterpret_cast<float*>(b);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        __hip_atomic_store(&f[k], v.lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&f[3 + k], v.hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+__device__ __forceinline__ Box load_box_agent(const Box* b) {
+    const float* f = reinterpret_cast<const float*>(b);
     Box r;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        r.lo[k] = f[k];
-        r.hi[k] = f[3 + k];
+        r.lo[k] = __hip_atomic_load(&f[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r.hi[k] = __hip_atomic_load(&f[3 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return r;
 }
 
+// Bottom-up refit: one thread per leaf climbs; at each internal node the first
+// of the two children to arrive stops, the second (which finds the sibling's
+// box published) writes the node and goes on with the union.
 __global__ void __launch_bounds__(256) k_refit(uint64_t n, const int32_t* __restrict__ parent_leaf,
                                                const int32_t* __restrict__ parent_int,
                                                const int32_t* __restrict__ left, const int32_t* __restrict__ right,
@@ -277,19 +312,21 @@ __global__ void __launch_bounds__(256) k_refit(uint64_t n, const int32_t* __rest
     if (i >= n) return;
     int32_t p = parent_leaf[i];
     for (int level = 0; p >= 0 && level < 128; ++level) {  // depth <= 64 in a Karras tree
-        __threadfence();
-        if (atomicAdd(&visit[p], 1u) == 0u) return;  // the sibling subtree is not finished yet
-        __threadfence();
-        int32_t l = left[p], r = right[p];
-        Box lb = l < 0 ? leaf_box[~l] : load_box_coherent(&node_box[l]);
-        Box rb = r < 0 ? leaf_box[~r] : load_box_coherent(&node_box[r]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this thread's box stores are complete
+        if (__hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            return;  // the sibling subtree is not finished yet
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int32_t l = left[p], r = right[p];
+        // leaf boxes come from an earlier kernel; internal ones were published above
+        const Box lb = l < 0 ? leaf_box[~l] : load_box_agent(&node_box[l]);
+        const Box rb = r < 0 ? leaf_box[~r] : load_box_agent(&node_box[r]);
         Box u;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             u.lo[k] = fminf(lb.lo[k], rb.lo[k]);
             u.hi[k] = fmaxf(lb.hi[k], rb.hi[k]);
         }
-        node_box[p] = u;
+        store_box_agent(&node_box[p], u);
         BvhNode nd;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -320,13 +357,13 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
     if (n >= (1ull << 29)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "too many triangles for one LBVH");
     bvh.leaves.reset(n);
     bvh.nodes.reset(n > 1 ? n - 1 : 1);
-    uint32_t* bounds = A.alloc<uint32_t>(6);
+    uint32_t* bounds = A.alloc<uint32_t>(6 * BOUND_STRIDE);
     uint32_t* keys = A.alloc<uint32_t>(n);
     uint32_t* vals = A.alloc<uint32_t>(n);
     Box* leaf_box = A.alloc<Box>(n);
     SDFX_LAUNCH(k_init_bounds, dim3(1), dim3(64), 0, s, bounds);
     unsigned bb = blocks_for(n, 256);
-    SDFX_LAUNCH(k_centroid_bounds, dim3(bb > 1024 ? 1024 : bb), dim3(256), 0, s, d_tris, n, bounds);
+    SDFX_LAUNCH(k_centroid_bounds, dim3(bb > 512 ? 512 : bb), dim3(256), 0, s, d_tris, n, bounds);
     SDFX_LAUNCH(k_morton, dim3(bb), dim3(256), 0, s, d_tris, n, bounds, keys, vals);
     SDFX_HIP(hipGetLastError());
     radix_sort_pairs(keys, vals, n, 30, A);
