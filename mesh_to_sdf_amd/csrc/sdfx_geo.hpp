@@ -293,4 +293,23 @@ SDFX_HD float box_dist2(float px, float py, float pz, const float* lo, const flo
     return dx * dx + dy * dy + dz * dz;
 }
 
+// Pruning by box distance keeps a box when box_d2 * PRUNE_SLACK <= best_d2:
+// the 2^-16 slack absorbs the rounding of the box distance, so a box is only
+// dropped when every point in it is clearly farther than the best distance
+// found, and the nearest triangle (minimum FP32 d^2) is never missed.
+constexpr float PRUNE_SLACK = 1.0f - 1.0f / 65536.0f;
+
+// Axes (bit k = +k ray) whose ray from p can meet the box: the two
+// perpendicular coordinates inside, the box not entirely behind p.
+SDFX_HD uint32_t ray_axes(const float* lo, const float* hi, float px, float py, float pz) {
+    bool ix = lo[0] <= px && px <= hi[0];
+    bool iy = lo[1] <= py && py <= hi[1];
+    bool iz = lo[2] <= pz && pz <= hi[2];
+    uint32_t m = 0;
+    if (iy && iz && hi[0] >= px) m |= 1u;
+    if (iz && ix && hi[1] >= py) m |= 2u;
+    if (ix && iy && hi[2] >= pz) m |= 4u;
+    return m;
+}
+
 }  // namespace sdfx

@@ -392,7 +392,6 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
 // ------------------------------------------------------------------ traversal
 
 constexpr int STACK_DEPTH = 64;  // Karras depth <= 64 (32 code bits + 32 position bits)
-constexpr float PRUNE_SLACK = 1.0f - 1.0f / 65536.0f;
 
 // Traversal stack of one lane: the newest STACK_RING entries live in LDS (a
 // ring in the lane's own column, so lanes never share words and need no
@@ -598,19 +597,6 @@ __device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec
         if (!found) break;
     }
     return acc;
-}
-
-// Axes (bit k = +k ray) whose ray from p can meet the box: the two
-// perpendicular coordinates inside, the box not entirely behind p.
-__device__ __forceinline__ uint32_t ray_axes(const float* lo, const float* hi, float px, float py, float pz) {
-    bool ix = lo[0] <= px && px <= hi[0];
-    bool iy = lo[1] <= py && py <= hi[1];
-    bool iz = lo[2] <= pz && pz <= hi[2];
-    uint32_t m = 0;
-    if (iy && iz && hi[0] >= px) m |= 1u;
-    if (iz && ix && hi[1] >= py) m |= 2u;
-    if (ix && iy && hi[2] >= pz) m |= 4u;
-    return m;
 }
 
 __device__ __forceinline__ void ray_leaf(const TriRec* __restrict__ leaves, int32_t leaf, uint32_t axes, float px,

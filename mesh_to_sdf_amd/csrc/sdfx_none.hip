@@ -125,17 +125,26 @@ __global__ void __launch_bounds__(NONE_THREADS) k_exhaustive(const TriRec* __res
     constexpr bool WANT_PARITY = (KIND & 2) != 0;
     for (uint64_t t = t0; t < t1; ++t) {
         const TriRec tr = tris[t];
-        float dx, dy, dz, cx, cy, cz;
-        closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
-        float d2 = dot3(dx, dy, dz, dx, dy, dz);
-        if (d2 < bd) {  // ascending t: the first (lowest index) of equal minima stays
-            bd = d2;
-            bi = (uint32_t)t;
+        // The tests the LBVH traversal makes at its leaves: the exact distance
+        // only where the padded box is not clearly farther than the best so far
+        // (such a triangle cannot be the first of the minima), a ray only where
+        // it meets the padded box (a hit point lies inside it).
+        Box b;
+        tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
+        if (box_dist2(px, py, pz, b.lo, b.hi) * PRUNE_SLACK <= bd) {
+            float dx, dy, dz, cx, cy, cz;
+            closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
+            float d2 = dot3(dx, dy, dz, dx, dy, dz);
+            if (d2 < bd) {  // ascending t: the first (lowest index) of equal minima stays
+                bd = d2;
+                bi = (uint32_t)t;
+            }
         }
         if (WANT_PARITY) {
-            par ^= ray_hit<0>(px, py, pz, tr, nullptr) ? 1u : 0u;
-            par ^= ray_hit<1>(px, py, pz, tr, nullptr) ? 2u : 0u;
-            par ^= ray_hit<2>(px, py, pz, tr, nullptr) ? 4u : 0u;
+            const uint32_t axes = ray_axes(b.lo, b.hi, px, py, pz);
+            if ((axes & 1u) && ray_hit<0>(px, py, pz, tr, nullptr)) par ^= 1u;
+            if ((axes & 2u) && ray_hit<1>(px, py, pz, tr, nullptr)) par ^= 2u;
+            if ((axes & 4u) && ray_hit<2>(px, py, pz, tr, nullptr)) par ^= 4u;
         }
     }
     if (!active) return;
