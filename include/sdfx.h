@@ -263,6 +263,23 @@ sdfx_status sdfx_server_add_mesh(sdfx_server* server, const float* d_vertices, u
                                  const uint32_t* d_indices, uint64_t index_count, sdfx_topology topology,
                                  uint32_t* mesh_id_out);
 
+/* One mesh of sdfx_server_add_meshes: device pointers in, id and status out. */
+typedef struct sdfx_server_mesh {
+    const float* vertices;
+    uint64_t vertex_count;
+    const uint32_t* indices; /* NULL: no index buffer */
+    uint64_t index_count;
+    sdfx_topology topology;
+    uint32_t id;        /* set on success */
+    sdfx_status status; /* set by the call */
+} sdfx_server_mesh;
+
+/* sdfx_server_add_mesh for many meshes, admitted in order while they fit the
+ * budget, with their LBVHs built in one batched device build. Returns SDFX_OK
+ * when all were added, else the first failing mesh's status (each mesh's own
+ * status and id are filled in; sdfx_last_error names the mesh). */
+sdfx_status sdfx_server_add_meshes(sdfx_server* server, sdfx_server_mesh* meshes, uint64_t count);
+
 /* Drops a mesh from the server (requests already queued for it still run). */
 sdfx_status sdfx_server_remove_mesh(sdfx_server* server, uint32_t mesh_id);
 

@@ -44,6 +44,13 @@ class GridRequest(ctypes.Structure):
                 ("status", ctypes.c_int)]
 
 
+class ServerMesh(ctypes.Structure):
+    """sdfx_server_mesh (include/sdfx.h)."""
+    _fields_ = [("vertices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint64), ("indices", ctypes.c_void_p),
+                ("index_count", ctypes.c_uint64), ("topology", ctypes.c_int), ("id", ctypes.c_uint32),
+                ("status", ctypes.c_int)]
+
+
 class ServerStats(ctypes.Structure):
     """sdfx_server_stats (include/sdfx.h)."""
     _fields_ = [(n, ctypes.c_uint64) for n in ("requests_done", "batches", "max_batch_requests", "resident_bytes",
@@ -69,6 +76,7 @@ SIGNATURES = {
     "sdfx_server_create": (_i, [_u64v, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "sdfx_server_destroy": (None, [_vp]),
     "sdfx_server_add_mesh": (_i, [_vp, _vp, _u64v, _vp, _u64v, _i, _u32]),
+    "sdfx_server_add_meshes": (_i, [_vp, ctypes.POINTER(ServerMesh), _u64v]),
     "sdfx_server_remove_mesh": (_i, [_vp, ctypes.c_uint32]),
     "sdfx_server_submit_points": (_i, [_vp, ctypes.c_uint32, _vp, _u64v, _i, _i, _vp, _u64]),
     "sdfx_server_submit_grid": (_i, [_vp, ctypes.c_uint32, _f, _f, _u64, _i, _vp, _u64]),

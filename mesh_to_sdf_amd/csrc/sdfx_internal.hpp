@@ -60,6 +60,17 @@ struct Lbvh {
 // sort, Karras hierarchy, bottom-up refit of the padded triangle boxes.
 void lbvh_build(const TriRec* d_tris, uint64_t n_tris, Lbvh& bvh, Arena& arena);
 
+// Builds the LBVHs of many meshes in shared launches (one sort over all of
+// them, keyed by mesh then Morton code; with more than 4 meshes the Morton
+// code keeps its top 32 - ceil(log2 meshes) bits). Each mesh gets its own node
+// and leaf buffers, so it can be freed on its own.
+struct LbvhBuildJob {
+    const TriRec* tris;  // topology order
+    uint64_t n;
+    Lbvh* out;
+};
+void lbvh_build_batch(const std::vector<LbvhBuildJob>& jobs, Arena& arena);
+
 enum QueryMode : int {
     QM_RAYCAST_VOTE = 0,     // min distance (clamped to f32::MAX) + 3-ray vote: Bvh(Raycast)
     QM_NORMAL_FOLD = 1,      // compare_distances fold: Bvh(Normal)

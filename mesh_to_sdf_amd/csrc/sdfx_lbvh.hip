@@ -38,21 +38,40 @@ __device__ __forceinline__ void tri_centroid(const TriRec& t, float* c) {
     for (int k = 0; k < 3; ++k) c[k] = 0.5f * (b.lo[k] + b.hi[k]);
 }
 
-// The six bound words (min x, y, z as ordered ints, then max) sit 64 u32
-// (256 bytes) apart: device-scope atomics on one cache line serialise.
-constexpr uint32_t BOUND_STRIDE = 64;
+// One mesh of a (batched) LBVH build. Kernels run on 2-D grids: blockIdx.y is
+// the mesh, blockIdx.x * blockDim.x + threadIdx.x the triangle (or internal
+// node) within it; blocks past a small mesh's end exit at once.
+struct BuildSeg {
+    const TriRec* tris;  // the mesh's triangles in topology order
+    TriRec* leaves;      // out: n triangles in Morton order
+    BvhNode* nodes;      // out: n - 1 internal nodes
+    uint64_t tri_off;    // first triangle (and sorted position) of the mesh in the batch
+    uint64_t node_off;   // first internal node of the mesh in the batch
+    uint64_t n;
+};
 
-__global__ void k_init_bounds(uint32_t* ob) {
-    if (threadIdx.x < 3) ob[threadIdx.x * BOUND_STRIDE] = 0xFFFFFFFFu;
-    else if (threadIdx.x < 6) ob[threadIdx.x * BOUND_STRIDE] = 0u;
+// Per mesh six bound words (min x, y, z as ordered ints, then max), each 64
+// u32 (256 bytes) apart: device-scope atomics on one cache line serialise.
+constexpr uint32_t BOUND_STRIDE = 64;
+constexpr uint32_t BOUND_WORDS = 6 * BOUND_STRIDE;
+// Centroid-bound blocks each reduce this many triangles of one mesh.
+constexpr uint32_t BOUND_CHUNK = 256 * 16;
+
+__global__ void k_init_bounds(uint32_t* ob, uint32_t nseg) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)nseg * 6) return;
+    ob[(i / 6) * BOUND_WORDS + (i % 6) * BOUND_STRIDE] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
 }
 
-__global__ void __launch_bounds__(256) k_centroid_bounds(const TriRec* __restrict__ tris, uint64_t n,
-                                                         uint32_t* __restrict__ ob) {
+__global__ void __launch_bounds__(256) k_centroid_bounds(const BuildSeg* __restrict__ segs, uint32_t* __restrict__ ob) {
+    const BuildSeg& S = segs[blockIdx.y];
+    const uint64_t c0 = (uint64_t)blockIdx.x * BOUND_CHUNK;
+    if (c0 >= S.n) return;  // block-uniform
+    const uint64_t c1 = min(S.n, c0 + BOUND_CHUNK);
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
         float c[3];
-        tri_centroid(tris[i], c);
+        tri_centroid(S.tris[i], c);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             uint32_t o = f2ord(c[k]);
@@ -80,7 +99,7 @@ __global__ void __launch_bounds__(256) k_centroid_bounds(const TriRec* __restric
     __syncthreads();
     if (threadIdx.x < 6) {
         const int k = threadIdx.x % 3;
-        uint32_t* w = &ob[threadIdx.x * BOUND_STRIDE];
+        uint32_t* w = &ob[(uint64_t)blockIdx.y * BOUND_WORDS + threadIdx.x * BOUND_STRIDE];
         if (threadIdx.x < 3) {
             const uint32_t m = min(min(wmn[0][k], wmn[1][k]), min(wmn[2][k], wmn[3][k]));
             if (m < __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(w, m);
@@ -100,24 +119,31 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     return v;
 }
 
-__global__ void __launch_bounds__(256) k_morton(const TriRec* __restrict__ tris, uint64_t n,
-                                                const uint32_t* __restrict__ ob, uint32_t* __restrict__ keys,
+// Sort key of each triangle: its mesh in the top seg_bits bits, then the top
+// morton_bits bits of its 30-bit Morton code (centroid in the mesh's centroid
+// box, 10 bits per axis). With one mesh the key is the full Morton code.
+__global__ void __launch_bounds__(256) k_morton(const BuildSeg* __restrict__ segs, const uint32_t* __restrict__ ob,
+                                                int morton_bits, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ vals) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const BuildSeg& S = segs[blockIdx.y];
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const uint32_t* b = ob + (uint64_t)blockIdx.y * BOUND_WORDS;
     float c[3];
-    tri_centroid(tris[i], c);
+    tri_centroid(S.tris[i], c);
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float lo = ord2f(ob[k * BOUND_STRIDE]), hi = ord2f(ob[(3 + k) * BOUND_STRIDE]);
+        float lo = ord2f(b[k * BOUND_STRIDE]), hi = ord2f(b[(3 + k) * BOUND_STRIDE]);
         float ext = hi - lo;
         float u = ext > 0.0f ? (c[k] - lo) / ext : 0.0f;
-        float s = u * 1024.0f;
-        q[k] = s <= 0.0f ? 0u : (s >= 1023.0f ? 1023u : (uint32_t)s);  // NaN -> 0
+        float sc = u * 1024.0f;
+        q[k] = sc <= 0.0f ? 0u : (sc >= 1023.0f ? 1023u : (uint32_t)sc);  // NaN -> 0
     }
-    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-    vals[i] = (uint32_t)i;
+    const uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    const uint32_t seg_part = morton_bits < 32 ? (uint32_t)blockIdx.y << morton_bits : 0u;
+    keys[S.tri_off + i] = seg_part | (code >> (30 - morton_bits));
+    vals[S.tri_off + i] = (uint32_t)(S.tri_off + i);
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -232,11 +258,21 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
     return 32 + __clz((uint32_t)(i ^ j));
 }
 
-__global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ keys, int64_t n,
-                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
-                                                int32_t* __restrict__ parent_int, int32_t* __restrict__ parent_leaf) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Karras 2012 on one mesh's sorted keys (keys[0, n) of that mesh); the
+// outputs are the mesh's own arrays, indices local to the mesh.
+__global__ void __launch_bounds__(256) k_karras(const BuildSeg* __restrict__ segs, const uint32_t* __restrict__ skeys,
+                                                int32_t* __restrict__ g_left, int32_t* __restrict__ g_right,
+                                                int32_t* __restrict__ g_parent_int,
+                                                int32_t* __restrict__ g_parent_leaf) {
+    const BuildSeg& S = segs[blockIdx.y];
+    const int64_t n = (int64_t)S.n;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
+    const uint32_t* keys = skeys + S.tri_off;
+    int32_t* left = g_left + S.node_off;
+    int32_t* right = g_right + S.node_off;
+    int32_t* parent_int = g_parent_int + S.node_off;
+    int32_t* parent_leaf = g_parent_leaf + S.tri_off;
     int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
     int dmin = delta(keys, n, i, i - d);
     int64_t lmax = 2;
@@ -264,15 +300,16 @@ __global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ key
     if (i == 0) parent_int[0] = -1;
 }
 
-__global__ void __launch_bounds__(256) k_leaves(const TriRec* __restrict__ tris, const uint32_t* __restrict__ order,
-                                                uint64_t n, TriRec* __restrict__ leaves, Box* __restrict__ leaf_box) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    TriRec t = tris[order[i]];
-    leaves[i] = t;
+__global__ void __launch_bounds__(256) k_leaves(const BuildSeg* __restrict__ segs, const uint32_t* __restrict__ order,
+                                                Box* __restrict__ g_leaf_box) {
+    const BuildSeg& S = segs[blockIdx.y];
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const TriRec t = S.tris[order[S.tri_off + i] - S.tri_off];
+    S.leaves[i] = t;
     Box b;
     tri_bbox(t.ax, t.ay, t.az, t.bx, t.by, t.bz, t.cx, t.cy, t.cz, b);
-    leaf_box[i] = b;
+    g_leaf_box[S.tri_off + i] = b;
 }
 
 // Boxes handed from one thread to another (possibly on another XCD) are
@@ -303,14 +340,22 @@ __device__ __forceinline__ Box load_box_agent(const Box* b) {
 // Bottom-up refit: one thread per leaf climbs; at each internal node the first
 // of the two children to arrive stops, the second (which finds the sibling's
 // box published) writes the node and goes on with the union.
-__global__ void __launch_bounds__(256) k_refit(uint64_t n, const int32_t* __restrict__ parent_leaf,
-                                               const int32_t* __restrict__ parent_int,
-                                               const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                                               const Box* __restrict__ leaf_box, Box* node_box, uint32_t* visit,
-                                               BvhNode* __restrict__ nodes) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t p = parent_leaf[i];
+__global__ void __launch_bounds__(256) k_refit(const BuildSeg* __restrict__ segs,
+                                               const int32_t* __restrict__ g_parent_leaf,
+                                               const int32_t* __restrict__ g_parent_int,
+                                               const int32_t* __restrict__ g_left, const int32_t* __restrict__ g_right,
+                                               const Box* __restrict__ g_leaf_box, Box* g_node_box, uint32_t* g_visit) {
+    const BuildSeg& S = segs[blockIdx.y];
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (S.n < 2 || i >= S.n) return;
+    const int32_t* parent_int = g_parent_int + S.node_off;
+    const int32_t* left = g_left + S.node_off;
+    const int32_t* right = g_right + S.node_off;
+    const Box* leaf_box = g_leaf_box + S.tri_off;
+    Box* node_box = g_node_box + S.node_off;
+    uint32_t* visit = g_visit + S.node_off;
+    BvhNode* nodes = S.nodes;
+    int32_t p = g_parent_leaf[S.tri_off + i];
     for (int level = 0; p >= 0 && level < 128; ++level) {  // depth <= 64 in a Karras tree
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this thread's box stores are complete
         if (__hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
@@ -344,49 +389,62 @@ __global__ void __launch_bounds__(256) k_refit(uint64_t n, const int32_t* __rest
     }
 }
 
-__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+void lbvh_build_batch(const std::vector<LbvhBuildJob>& jobs, Arena& A) {
+    hipStream_t s = A.stream;
+    std::vector<BuildSeg> segs;
+    uint64_t total = 0, total_nodes = 0, max_n = 0;
+    for (const LbvhBuildJob& J : jobs) {
+        Lbvh& bvh = *J.out;
+        bvh.n = J.n;
+        if (J.n == 0) continue;
+        // ray-traversal stack entries pack a node index with 3 axis bits
+        if (J.n >= (1ull << 29)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "too many triangles for one LBVH");
+        bvh.leaves.reset(J.n);
+        bvh.nodes.reset(J.n > 1 ? J.n - 1 : 1);
+        bvh.root = J.n > 1 ? 0 : ~0;
+        segs.push_back(BuildSeg{J.tris, bvh.leaves.get(), bvh.nodes.get(), total, total_nodes, J.n});
+        total += J.n;
+        total_nodes += J.n - 1;
+        max_n = std::max(max_n, J.n);
+    }
+    if (segs.empty()) return;
+    if (total >= (1ull << 32) || segs.size() > 65535)
+        throw Error(SDFX_ERR_INVALID_ARGUMENT, "an LBVH build batch holds fewer than 2^32 triangles and 65,536 meshes");
+    const uint32_t nseg = (uint32_t)segs.size();
+    int seg_bits = 0;  // the mesh field of the sort key
+    while ((1u << seg_bits) < nseg) ++seg_bits;
+    const int morton_bits = std::min(30, 32 - seg_bits);
+    BuildSeg* d_segs = A.alloc<BuildSeg>(nseg);
+    SDFX_HIP(hipMemcpyAsync(d_segs, segs.data(), nseg * sizeof(BuildSeg), hipMemcpyHostToDevice, s));
+    uint32_t* bounds = A.alloc<uint32_t>((uint64_t)nseg * BOUND_WORDS);
+    uint32_t* keys = A.alloc<uint32_t>(total);
+    uint32_t* vals = A.alloc<uint32_t>(total);
+    Box* leaf_box = A.alloc<Box>(total);
+    const unsigned per_mesh = blocks_for(max_n, 256);
+    SDFX_LAUNCH(k_init_bounds, dim3(blocks_for(6ull * nseg, 256)), dim3(256), 0, s, bounds, nseg);
+    SDFX_LAUNCH(k_centroid_bounds, dim3(blocks_for(max_n, BOUND_CHUNK), nseg), dim3(256), 0, s, d_segs, bounds);
+    SDFX_LAUNCH(k_morton, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, bounds, morton_bits, keys, vals);
+    SDFX_HIP(hipGetLastError());
+    // sorted by (mesh, Morton code): mesh m's triangles land on [tri_off, tri_off + n)
+    radix_sort_pairs(keys, vals, total, seg_bits + morton_bits, A);
+    SDFX_LAUNCH(k_leaves, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, vals, leaf_box);
+    SDFX_HIP(hipGetLastError());
+    if (total_nodes == 0) return;  // single-triangle meshes only
+    int32_t* left = A.alloc<int32_t>(total_nodes);
+    int32_t* right = A.alloc<int32_t>(total_nodes);
+    int32_t* parent_int = A.alloc<int32_t>(total_nodes);
+    int32_t* parent_leaf = A.alloc<int32_t>(total);
+    Box* node_box = A.alloc<Box>(total_nodes);
+    uint32_t* visit = A.alloc<uint32_t>(total_nodes);
+    fill_u32(visit, total_nodes, 0u, s);
+    SDFX_LAUNCH(k_karras, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, keys, left, right, parent_int, parent_leaf);
+    SDFX_LAUNCH(k_refit, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, parent_leaf, parent_int, left, right,
+                leaf_box, node_box, visit);
+    SDFX_HIP(hipGetLastError());
 }
 
 void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
-    hipStream_t s = A.stream;
-    bvh.n = n;
-    if (n == 0) return;
-    // ray-traversal stack entries pack a node index with 3 axis bits
-    if (n >= (1ull << 29)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "too many triangles for one LBVH");
-    bvh.leaves.reset(n);
-    bvh.nodes.reset(n > 1 ? n - 1 : 1);
-    uint32_t* bounds = A.alloc<uint32_t>(6 * BOUND_STRIDE);
-    uint32_t* keys = A.alloc<uint32_t>(n);
-    uint32_t* vals = A.alloc<uint32_t>(n);
-    Box* leaf_box = A.alloc<Box>(n);
-    SDFX_LAUNCH(k_init_bounds, dim3(1), dim3(64), 0, s, bounds);
-    unsigned bb = blocks_for(n, 256);
-    SDFX_LAUNCH(k_centroid_bounds, dim3(bb > 512 ? 512 : bb), dim3(256), 0, s, d_tris, n, bounds);
-    SDFX_LAUNCH(k_morton, dim3(bb), dim3(256), 0, s, d_tris, n, bounds, keys, vals);
-    SDFX_HIP(hipGetLastError());
-    radix_sort_pairs(keys, vals, n, 30, A);
-    SDFX_LAUNCH(k_leaves, dim3(bb), dim3(256), 0, s, d_tris, vals, n, bvh.leaves.get(), leaf_box);
-    SDFX_HIP(hipGetLastError());
-    if (n == 1) {
-        bvh.root = ~0;
-        return;
-    }
-    bvh.root = 0;
-    uint64_t ni = n - 1;
-    int32_t* left = A.alloc<int32_t>(ni);
-    int32_t* right = A.alloc<int32_t>(ni);
-    int32_t* parent_int = A.alloc<int32_t>(ni);
-    int32_t* parent_leaf = A.alloc<int32_t>(n);
-    Box* node_box = A.alloc<Box>(ni);
-    uint32_t* visit = A.alloc<uint32_t>(ni);
-    SDFX_LAUNCH(k_zero_u32, dim3(blocks_for(ni, 256)), dim3(256), 0, s, visit, ni);
-    SDFX_LAUNCH(k_karras, dim3(blocks_for(ni, 256)), dim3(256), 0, s, keys, (int64_t)n, left, right,
-                       parent_int, parent_leaf);
-    SDFX_LAUNCH(k_refit, dim3(bb), dim3(256), 0, s, n, parent_leaf, parent_int, left, right, leaf_box,
-                       node_box, visit, bvh.nodes.get());
-    SDFX_HIP(hipGetLastError());
+    lbvh_build_batch({LbvhBuildJob{d_tris, n, &bvh}}, A);
 }
 
 // ------------------------------------------------------------------ traversal

@@ -319,67 +319,148 @@ void sdfx_server_destroy(sdfx_server* sv) {
     delete sv;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Adds meshes[0, count): validates each, admits them in order while they fit
+// the budget beside the resident meshes (then waits for the requests in flight
+// to leave room), gathers their triangles and builds all their LBVHs in one
+// batched build. Fills each mesh's id and status; message[i] holds the
+// reference's message for a failed one.
+void add_meshes_core(sdfx_server* sv, sdfx_server_mesh* meshes, uint64_t count, std::vector<std::string>& message) {
+    message.assign(count, std::string());
+    std::vector<uint64_t> nt(count, 0), bytes(count, 0);
+    for (uint64_t i = 0; i < count; ++i) {
+        sdfx_server_mesh& d = meshes[i];
+        d.id = 0;
+        d.status = SDFX_OK;
+        if (!d.vertices && d.vertex_count) {
+            d.status = SDFX_ERR_INVALID_ARGUMENT;
+            message[i] = "null pointer: vertices";
+        } else if (d.topology != SDFX_TRIANGLE_LIST && d.topology != SDFX_TRIANGLE_STRIP) {
+            d.status = SDFX_ERR_INVALID_ARGUMENT;
+            message[i] = "unknown topology";
+        } else {
+            const sdfx_status ts =
+                topology_triangle_count(d.vertex_count, d.indices != nullptr, d.index_count, d.topology, &nt[i]);
+            if (ts == SDFX_ERR_INDEX_COUNT)
+                message[i] = "TriangleList indices length (" + std::to_string(d.index_count) + ") must be divisible by 3";
+            else if (ts == SDFX_ERR_VERTEX_COUNT)
+                message[i] = "TriangleList vertices length (" + std::to_string(d.vertex_count) +
+                             ") must be divisible by 3 when no indices are provided";
+            else if (ts != SDFX_OK)
+                message[i] = "invalid topology";
+            d.status = ts;
+        }
+        bytes[i] = nt[i] * MESH_BYTES_PER_TRI + REQUEST_BYTES;
+    }
+    uint64_t reserved = 0;
+    {
+        // admitted while they fit beside the resident meshes; then the requests
+        // in flight (which always drain) must leave room
+        std::unique_lock<std::mutex> lk(sv->mu);
+        for (uint64_t i = 0; i < count; ++i) {
+            if (meshes[i].status != SDFX_OK) continue;
+            if (sv->resident + reserved + bytes[i] > sv->budget) {
+                meshes[i].status = SDFX_ERR_OVER_BUDGET;
+                message[i] = "mesh needs " + std::to_string(bytes[i]) + " bytes; " +
+                             std::to_string(sv->budget - std::min(sv->budget, sv->resident + reserved)) +
+                             " of the budget are not held by meshes";
+                continue;
+            }
+            reserved += bytes[i];
+        }
+        sv->cv_done.wait(lk, [&] { return sv->resident + sv->inflight + reserved <= sv->budget; });
+        sv->resident += reserved;
+        sv->stats.resident_bytes = sv->resident;
+    }
+    auto release = [&](uint64_t b) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->resident -= b;
+        sv->stats.resident_bytes = sv->resident;
+    };
+    std::vector<std::shared_ptr<ServerMesh>> built(count);
+    try {
+        SDFX_HIP(hipSetDevice(sv->device));
+        hipStream_t s = thread_stream();
+        std::vector<DevFlags> hf(count ? count : 1);
+        {
+            Arena A(s);
+            DevFlags* flags = A.alloc<DevFlags>(count ? count : 1);
+            fill_u32(reinterpret_cast<uint32_t*>(flags), (count ? count : 1) * (sizeof(DevFlags) / sizeof(uint32_t)),
+                     0u, s);
+            std::vector<LbvhBuildJob> jobs;
+            for (uint64_t i = 0; i < count; ++i) {
+                if (meshes[i].status != SDFX_OK) continue;
+                const sdfx_server_mesh& d = meshes[i];
+                auto m = std::make_shared<ServerMesh>();
+                m->nv = d.vertex_count;
+                m->nt = nt[i];
+                m->bytes = bytes[i];
+                m->tris.reset(nt[i]);
+                gather_triangles(d.vertices, d.vertex_count, d.indices, d.topology, nt[i], m->tris.get(), flags + i,
+                                 s);
+                if (nt[i] > 0) jobs.push_back(LbvhBuildJob{m->tris.get(), nt[i], &m->bvh});
+                built[i] = std::move(m);
+            }
+            lbvh_build_batch(jobs, A);
+            SDFX_HIP(hipMemcpyAsync(hf.data(), flags, count * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
+            SDFX_HIP(hipStreamSynchronize(s));
+        }
+        for (uint64_t i = 0; i < count; ++i) {
+            if (!built[i] || !hf[i].index_out_of_range) continue;
+            meshes[i].status = SDFX_ERR_INDEX_OUT_OF_RANGE;
+            message[i] = "index out of bounds: the len is " + std::to_string(meshes[i].vertex_count) +
+                         " but a triangle index is >= " + std::to_string(meshes[i].vertex_count);
+            release(bytes[i]);
+            built[i].reset();
+        }
+    } catch (...) {
+        uint64_t held = 0;
+        for (uint64_t i = 0; i < count; ++i)
+            if (built[i] || meshes[i].status == SDFX_OK) held += bytes[i];
+        release(held);
+        throw;
+    }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (!built[i]) continue;
+        const uint32_t id = sv->next_mesh++;
+        sv->meshes[id] = std::move(built[i]);
+        meshes[i].id = id;
+    }
+    sv->stats.meshes = sv->meshes.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+sdfx_status sdfx_server_add_meshes(sdfx_server* sv, sdfx_server_mesh* meshes, uint64_t count) {
+    return server_guarded([&]() -> sdfx_status {
+        if (!sv || (!meshes && count)) server_fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: server or meshes");
+        std::vector<std::string> message;
+        add_meshes_core(sv, meshes, count, message);
+        for (uint64_t i = 0; i < count; ++i)
+            if (meshes[i].status != SDFX_OK) {
+                set_last_error("mesh " + std::to_string(i) + ": " + message[i]);
+                return meshes[i].status;
+            }
+        return SDFX_OK;
+    });
+}
+
 sdfx_status sdfx_server_add_mesh(sdfx_server* sv, const float* d_vertices, uint64_t vertex_count,
                                  const uint32_t* d_indices, uint64_t index_count, sdfx_topology topology,
                                  uint32_t* mesh_id_out) {
     return server_guarded([&]() -> sdfx_status {
         if (!sv || !mesh_id_out) server_fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: server or mesh_id_out");
-        if (!d_vertices && vertex_count) server_fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: vertices");
-        if (topology != SDFX_TRIANGLE_LIST && topology != SDFX_TRIANGLE_STRIP)
-            server_fail(SDFX_ERR_INVALID_ARGUMENT, "unknown topology");
-        uint64_t nt = 0;
-        const sdfx_status ts = topology_triangle_count(vertex_count, d_indices != nullptr, index_count, topology, &nt);
-        if (ts == SDFX_ERR_INDEX_COUNT)
-            server_fail(ts, "TriangleList indices length (" + std::to_string(index_count) + ") must be divisible by 3");
-        if (ts == SDFX_ERR_VERTEX_COUNT)
-            server_fail(ts, "TriangleList vertices length (" + std::to_string(vertex_count) +
-                                ") must be divisible by 3 when no indices are provided");
-        if (ts != SDFX_OK) server_fail(ts, "invalid topology");
-        const uint64_t bytes = nt * MESH_BYTES_PER_TRI + REQUEST_BYTES;
-        {
-            // admitted once the mesh fits beside the resident meshes and the
-            // requests in flight (which always drain)
-            std::unique_lock<std::mutex> lk(sv->mu);
-            if (sv->resident + bytes > sv->budget)
-                server_fail(SDFX_ERR_OVER_BUDGET, "mesh needs " + std::to_string(bytes) + " bytes; " +
-                                                      std::to_string(sv->budget - std::min(sv->budget, sv->resident)) +
-                                                      " of the budget are not held by meshes");
-            sv->cv_done.wait(lk, [&] { return sv->resident + sv->inflight + bytes <= sv->budget; });
-            sv->resident += bytes;
-            sv->stats.resident_bytes = sv->resident;
-        }
-        auto m = std::make_shared<ServerMesh>();
-        m->nv = vertex_count;
-        m->nt = nt;
-        m->bytes = bytes;
-        try {
-            SDFX_HIP(hipSetDevice(sv->device));
-            hipStream_t s = thread_stream();
-            m->tris.reset(nt);
-            Arena A(s);
-            DevFlags* flags = A.alloc<DevFlags>(1);
-            fill_u32(reinterpret_cast<uint32_t*>(flags), sizeof(DevFlags) / sizeof(uint32_t), 0u, s);
-            gather_triangles(d_vertices, vertex_count, d_indices, topology, nt, m->tris.get(), flags, s);
-            if (nt > 0) lbvh_build(m->tris.get(), nt, m->bvh, A);
-            DevFlags f{};
-            SDFX_HIP(hipMemcpyAsync(&f, flags, sizeof(DevFlags), hipMemcpyDeviceToHost, s));
-            SDFX_HIP(hipStreamSynchronize(s));
-            if (f.index_out_of_range)
-                server_fail(SDFX_ERR_INDEX_OUT_OF_RANGE, "index out of bounds: the len is " +
-                                                             std::to_string(vertex_count) +
-                                                             " but a triangle index is >= " +
-                                                             std::to_string(vertex_count));
-        } catch (...) {
-            std::lock_guard<std::mutex> lk(sv->mu);
-            sv->resident -= bytes;
-            sv->stats.resident_bytes = sv->resident;
-            throw;
-        }
-        std::lock_guard<std::mutex> lk(sv->mu);
-        const uint32_t id = sv->next_mesh++;
-        sv->meshes[id] = std::move(m);
-        sv->stats.meshes = sv->meshes.size();
-        *mesh_id_out = id;
+        sdfx_server_mesh d{d_vertices, vertex_count, d_indices, index_count, topology, 0u, SDFX_OK};
+        std::vector<std::string> message;
+        add_meshes_core(sv, &d, 1, message);
+        if (d.status != SDFX_OK) server_fail(d.status, message[0]);
+        *mesh_id_out = d.id;
         return SDFX_OK;
     });
 }

@@ -81,6 +81,26 @@ class SdfServer:
         self._torch.cuda.synchronize()
         return int(mid.value)
 
+    def add_meshes(self, meshes) -> list:
+        """Adds (vertices, topology) pairs with one batched LBVH build; returns
+        their ids (raises the first failing mesh's error)."""
+        t = self._torch
+        descs = (_lib.ServerMesh * max(1, len(meshes)))()
+        keep = []
+        for d, (vertices, indices) in zip(descs, meshes):
+            v = self._dev(np.asarray(vertices, dtype=np.float32).reshape(-1, 3), t.float32)
+            idx = None
+            if indices.indices is not None:
+                idx = self._dev(np.asarray(indices.indices, dtype=np.uint32).view(np.int32), t.int32)
+            keep += [v, idx]
+            d.vertices, d.vertex_count = v.data_ptr() if v.numel() else None, v.shape[0]
+            d.indices = None if idx is None or not idx.numel() else idx.data_ptr()
+            d.index_count = 0 if idx is None else idx.numel()
+            d.topology = indices.kind
+        _lib.check(_lib.lib().sdfx_server_add_meshes(self._h, descs, len(meshes)))
+        t.cuda.synchronize()
+        return [int(descs[i].id) for i in range(len(meshes))]
+
     def remove_mesh(self, mesh_id: int) -> None:
         _lib.check(_lib.lib().sdfx_server_remove_mesh(self._h, int(mesh_id)))
 

@@ -140,3 +140,33 @@ def test_failures_are_isolated(sdf):
         with pytest.raises(_lib.SdfxError) as e:
             srv.submit_points(a, q, sdf.AccelerationMethod.RtreeBvh)
         assert e.value.status == _lib.ERR_INVALID_ARGUMENT
+
+
+def test_batched_mesh_admission(sdf):
+    """Meshes added in one batched build answer exactly as meshes added one by
+    one (and as the plain entry points); a failing mesh of the batch does not
+    stop the others."""
+    from mesh_to_sdf_amd import _lib
+    meshes = _meshes() + [S.icosphere(1), (S.icosphere(0)[0], S.icosphere(0)[1][:1])]  # incl. one triangle
+    g = np.random.default_rng(21)
+    with _server() as one, _server() as many:
+        ids1 = [one.add_meshes([(v, sdf.Topology.TriangleList(f))])[0] for v, f in meshes]
+        ids2 = many.add_meshes([(v, sdf.Topology.TriangleList(f)) for v, f in meshes])
+        for k, (v, f) in enumerate(meshes):
+            q = S.uniform_queries(v, 3000, seed=300 + k)
+            for m in (sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast), sdf.AccelerationMethod.Bvh(sdf.SignMethod.Normal),
+                      sdf.AccelerationMethod.Rtree, sdf.AccelerationMethod.RtreeBvh):
+                a, _ = one.wait(one.submit_points(ids1[k], q, m))
+                b, _ = many.wait(many.submit_points(ids2[k], q, m))
+                ref = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, m)
+                assert np.array_equal(bits(a.cpu().numpy()), bits(b.cpu().numpy())), (k, m)
+                assert np.array_equal(bits(b.cpu().numpy()), bits(ref)), (k, m)
+        v, f = S.icosphere(2)
+        bad = f.reshape(-1).copy()
+        bad[3] = len(v) + 7
+        descs = [(v, sdf.Topology.TriangleList(f)), (v, sdf.Topology.TriangleList(bad)),
+                 (v, sdf.Topology.TriangleList(f))]
+        with pytest.raises(_lib.SdfxError) as e:
+            many.add_meshes(descs)
+        assert e.value.status == _lib.ERR_INDEX_OUT_OF_RANGE and "mesh 1" in e.value.message
+        assert many.stats()["meshes"] == len(meshes) + 2  # the two good ones of the batch were added
