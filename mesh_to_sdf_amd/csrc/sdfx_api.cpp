@@ -270,15 +270,17 @@ sdfx_status grid_batch_core(sdfx_grid_request* reqs, uint64_t n, sdfx_sign_metho
     DevFlags* flags = A.alloc<DevFlags>(n ? n : 1);
     fill_u32(reinterpret_cast<uint32_t*>(flags), (n ? n : 1) * (sizeof(DevFlags) / sizeof(uint32_t)), 0u, s);
     uint64_t off = 0;
+    std::vector<GatherJob> gathers;
     for (uint64_t i = 0; i < n; ++i) {
         if (prep[i].status != SDFX_OK) continue;
         const sdfx_grid_request& r = reqs[i];
-        gather_triangles(r.vertices, r.vertex_count, r.indices, r.topology, prep[i].nt, tris.get() + off, flags + i,
-                         s);
+        gathers.push_back(GatherJob{r.vertices, r.vertex_count, r.indices, r.topology, prep[i].nt, tris.get() + off,
+                                    flags + i});
         if (!(prep[i].zero_axis && prep[i].nt > 0))
             items.push_back(GridBatchItem{tris.get() + off, off, prep[i].nt, prep[i].g, r.out, (uint32_t)i});
         off += prep[i].nt;
     }
+    gather_triangles_batch(gathers, A);
     grid_sdf_batch(items, sign, flags, A);
     std::vector<DevFlags> hf(n ? n : 1);
     SDFX_HIP(hipMemcpyAsync(hf.data(), flags, (n ? n : 1) * sizeof(DevFlags), hipMemcpyDeviceToHost, s));

@@ -317,6 +317,32 @@ __global__ void __launch_bounds__(256) k_seed_boxes(uint64_t nt, ReqView<mode_ba
     counts[t] = cnt;
 }
 
+// The last t in [lo, hi] with offsets[t] <= target (offsets[lo] <= target),
+// searched by a whole wave: each step tests 64 evenly spaced positions, so a
+// range of n entries takes log64(n) dependent loads. All lanes of the wave
+// must call it with the same arguments; every lane gets the result.
+__device__ __forceinline__ uint64_t wave_search(const uint64_t* __restrict__ offsets, uint64_t lo, uint64_t hi,
+                                                uint64_t target) {
+    const uint32_t lane = threadIdx.x & 63u;
+    while (hi > lo) {
+        const uint64_t step = (hi - lo + 64) / 64;  // ceil((hi - lo + 1) / 64)
+        const uint64_t q = lo + lane * step;
+        const bool pred = q <= hi && offsets[q] <= target;  // true for lane 0
+        const uint64_t m = __ballot(pred);
+        const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
+        const uint64_t nlo = lo + last * step;
+        hi = min(hi, nlo + step - 1);
+        lo = nlo;
+    }
+    return lo;
+}
+
+// Seeding works in chunks of SEED_CHUNK consecutive (triangle, cell) pairs: a
+// wave finds the chunk's first and last triangles, the block loads their pair
+// offsets into LDS and each lane finds its pair's triangle there (a global
+// search only when zero-pair triangles make the range longer than the chunk).
+constexpr uint32_t SEED_CHUNK = 2048;
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_batch<MODE>> reqs,
                                                     const uint32_t* __restrict__ tri_req,
@@ -325,39 +351,59 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
                                                     const uint64_t* __restrict__ total_ptr, uint64_t* keys,
                                                     DevFlags* flags, unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
+    __shared__ uint64_t s_off[SEED_CHUNK];
+    __shared__ uint64_t s_t0, s_t1;
     const uint64_t total = *total_ptr;
     uint32_t n_evals = 0;
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
-         p += (uint64_t)gridDim.x * blockDim.x) {
-        // last triangle whose offset is <= p (the one whose pairs hold p)
-        uint64_t lo = 0, hi = nt - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (offsets[mid] <= p) lo = mid;
-            else hi = mid - 1;
+    for (uint64_t c0 = (uint64_t)blockIdx.x * SEED_CHUNK; c0 < total; c0 += (uint64_t)gridDim.x * SEED_CHUNK) {
+        const uint64_t c1 = min(total, c0 + SEED_CHUNK);
+        if (threadIdx.x < 64u) {
+            const uint64_t t0 = wave_search(offsets, 0, nt - 1, c0);
+            const uint64_t t1 = wave_search(offsets, t0, nt - 1, c1 - 1);
+            if (threadIdx.x == 0) {
+                s_t0 = t0;
+                s_t1 = t1;
+            }
         }
-        const uint64_t t = lo;
-        const GridReq& R = reqs[mode_batch<MODE> ? tri_req[t] : 0u];
-        const SeedBox sb = boxes[t];
-        uint64_t local = p - offsets[t];
-        uint64_t yz = (uint64_t)sb.ext[1] * sb.ext[2];
-        if (yz == 0 || local >= yz * sb.ext[0]) {  // cannot happen with a consistent prefix sum
-            atomicOr(&flags[R.flag].internal, 1u);
-            continue;
+        __syncthreads();
+        const uint64_t t0 = s_t0, span = s_t1 - t0 + 1;
+        const bool in_lds = span <= SEED_CHUNK;  // block-uniform
+        if (in_lds)
+            for (uint64_t k = threadIdx.x; k < span; k += blockDim.x) s_off[k] = offsets[t0 + k];
+        __syncthreads();
+        for (uint64_t p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
+            // last triangle whose offset is <= p (the one whose pairs hold p)
+            uint64_t lo = 0, hi = span - 1;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) >> 1;
+                if ((in_lds ? s_off[mid] : offsets[t0 + mid]) <= p) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint64_t t = t0 + lo;
+            const uint64_t off_t = in_lds ? s_off[lo] : offsets[t];
+            const GridReq& R = reqs[mode_batch<MODE> ? tri_req[t] : 0u];
+            const SeedBox sb = boxes[t];
+            uint64_t local = p - off_t;
+            uint64_t yz = (uint64_t)sb.ext[1] * sb.ext[2];
+            if (yz == 0 || local >= yz * sb.ext[0]) {  // cannot happen with a consistent prefix sum
+                atomicOr(&flags[R.flag].internal, 1u);
+                continue;
+            }
+            uint32_t x = sb.mn[0] + (uint32_t)(local / yz);
+            uint64_t r = local % yz;
+            uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
+            uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
+            const TriRec tr = R.tris[t - R.tri_off];
+            float d = cell_dist<SIGN>(R.g, x, y, z, tr);
+            n_evals++;
+            if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
+            if (key_valid(d)) {
+                uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
+                uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
+                if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+            }
         }
-        uint32_t x = sb.mn[0] + (uint32_t)(local / yz);
-        uint64_t r = local % yz;
-        uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
-        uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
-        const TriRec tr = R.tris[t - R.tri_off];
-        float d = cell_dist<SIGN>(R.g, x, y, z, tr);
-        n_evals++;
-        if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
-        if (key_valid(d)) {
-            uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
-            uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
-            if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
-        }
+        __syncthreads();  // s_off and s_t0 / s_t1 are reused by the next chunk
     }
     if (evals) block_count(evals, n_evals);
 }

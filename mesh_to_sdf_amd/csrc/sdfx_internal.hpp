@@ -23,6 +23,18 @@ void gather_triangles(const float* d_vertices, uint64_t n_vertices, const uint32
                       sdfx_topology topology, uint64_t n_tris, TriRec* d_tris, DevFlags* d_flags,
                       hipStream_t stream);
 
+// gather_triangles for many meshes in one launch.
+struct GatherJob {
+    const float* v;
+    uint64_t nv;
+    const uint32_t* idx;
+    sdfx_topology topo;
+    uint64_t nt;
+    TriRec* out;
+    DevFlags* flags;
+};
+void gather_triangles_batch(const std::vector<GatherJob>& jobs, Arena& arena);
+
 // Raises flags->non_finite_query if any coordinate is NaN or infinite (lib.rs:374-376).
 void check_queries_finite(const float* d_queries, uint64_t n_queries, DevFlags* d_flags, hipStream_t stream);
 

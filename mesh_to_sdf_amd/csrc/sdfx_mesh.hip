@@ -27,11 +27,11 @@ sdfx_status topology_triangle_count(uint64_t nv, bool has_idx, uint64_t ni, sdfx
     return SDFX_ERR_INVALID_ARGUMENT;
 }
 
-__global__ void __launch_bounds__(256) k_gather(const float* __restrict__ v, uint64_t nv,
-                                                const uint32_t* __restrict__ idx, int topo, uint64_t nt,
-                                                TriRec* __restrict__ out, DevFlags* flags) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
+// The triangle t of one mesh: its vertex indices (clamped, with the
+// out-of-range flag raised, so no read leaves the vertex buffer) and its
+// TriRec.
+__device__ __forceinline__ void gather_one(const float* __restrict__ v, uint64_t nv, const uint32_t* __restrict__ idx,
+                                           int topo, uint64_t t, TriRec* __restrict__ out, DevFlags* flags) {
     uint64_t base = topo == SDFX_TRIANGLE_LIST ? 3 * t : t;
     uint64_t vi[3];
     bool bad = false;
@@ -54,12 +54,45 @@ __global__ void __launch_bounds__(256) k_gather(const float* __restrict__ v, uin
     out[t] = r;
 }
 
+__global__ void __launch_bounds__(256) k_gather(const float* __restrict__ v, uint64_t nv,
+                                                const uint32_t* __restrict__ idx, int topo, uint64_t nt,
+                                                TriRec* __restrict__ out, DevFlags* flags) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) gather_one(v, nv, idx, topo, t, out, flags);
+}
+
+// Many meshes in one launch: blockIdx.y is the mesh.
+__global__ void __launch_bounds__(256) k_gather_batch(const GatherJob* __restrict__ jobs) {
+    const GatherJob& J = jobs[blockIdx.y];
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < J.nt) gather_one(J.v, J.nv, J.idx, (int)J.topo, t, J.out, J.flags);
+}
+
 void gather_triangles(const float* d_v, uint64_t nv, const uint32_t* d_idx, sdfx_topology topo, uint64_t nt,
                       TriRec* d_tris, DevFlags* d_flags, hipStream_t s) {
     if (!nt) return;
     SDFX_LAUNCH(k_gather, dim3(blocks_for(nt, 256)), dim3(256), 0, s, d_v, nv, d_idx, (int)topo, nt, d_tris,
                        d_flags);
     SDFX_HIP(hipGetLastError());
+}
+
+void gather_triangles_batch(const std::vector<GatherJob>& jobs, Arena& A) {
+    std::vector<GatherJob> live;
+    uint64_t max_nt = 0;
+    for (const GatherJob& J : jobs)
+        if (J.nt) {
+            live.push_back(J);
+            max_nt = std::max(max_nt, J.nt);
+        }
+    for (size_t c0 = 0; c0 < live.size(); c0 += 65535) {  // grid.y limit
+        const size_t c1 = std::min(live.size(), c0 + 65535);
+        GatherJob* d_jobs = A.alloc<GatherJob>(c1 - c0);
+        SDFX_HIP(hipMemcpyAsync(d_jobs, live.data() + c0, (c1 - c0) * sizeof(GatherJob), hipMemcpyHostToDevice,
+                                A.stream));
+        SDFX_LAUNCH(k_gather_batch, dim3(blocks_for(max_nt, 256), (unsigned)(c1 - c0)), dim3(256), 0, A.stream,
+                    d_jobs);
+        SDFX_HIP(hipGetLastError());
+    }
 }
 
 __global__ void __launch_bounds__(256) k_check_finite(const float* __restrict__ q, uint64_t n, DevFlags* flags) {

@@ -391,6 +391,7 @@ void add_meshes_core(sdfx_server* sv, sdfx_server_mesh* meshes, uint64_t count, 
             fill_u32(reinterpret_cast<uint32_t*>(flags), (count ? count : 1) * (sizeof(DevFlags) / sizeof(uint32_t)),
                      0u, s);
             std::vector<LbvhBuildJob> jobs;
+            std::vector<GatherJob> gathers;
             for (uint64_t i = 0; i < count; ++i) {
                 if (meshes[i].status != SDFX_OK) continue;
                 const sdfx_server_mesh& d = meshes[i];
@@ -399,11 +400,12 @@ void add_meshes_core(sdfx_server* sv, sdfx_server_mesh* meshes, uint64_t count, 
                 m->nt = nt[i];
                 m->bytes = bytes[i];
                 m->tris.reset(nt[i]);
-                gather_triangles(d.vertices, d.vertex_count, d.indices, d.topology, nt[i], m->tris.get(), flags + i,
-                                 s);
+                gathers.push_back(GatherJob{d.vertices, d.vertex_count, d.indices, d.topology, nt[i], m->tris.get(),
+                                            flags + i});
                 if (nt[i] > 0) jobs.push_back(LbvhBuildJob{m->tris.get(), nt[i], &m->bvh});
                 built[i] = std::move(m);
             }
+            gather_triangles_batch(gathers, A);
             lbvh_build_batch(jobs, A);
             SDFX_HIP(hipMemcpyAsync(hf.data(), flags, count * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
             SDFX_HIP(hipStreamSynchronize(s));
