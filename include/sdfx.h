@@ -62,7 +62,9 @@ typedef enum sdfx_status {
     /* grid.rs:311-315 clamp(0, n - 1) panics on a zero cell count */
     SDFX_ERR_INVALID_GRID = -10,
     /* sdfx_server_*: the mesh or request does not fit the server's HBM budget */
-    SDFX_ERR_OVER_BUDGET = -11
+    SDFX_ERR_OVER_BUDGET = -11,
+    /* sdfx_server_*: the request was cancelled before it ran */
+    SDFX_ERR_CANCELLED = -12
 } sdfx_status;
 
 /* lib.rs:156-171 Topology::{TriangleList, TriangleStrip}; indices == NULL is `None` */
@@ -299,6 +301,11 @@ sdfx_status sdfx_server_submit_grid(sdfx_server* server, uint32_t mesh_id, const
  * sdfx_last_error) and, when latency_ms is not NULL, the milliseconds from
  * submission to completion. A ticket can be waited for once. */
 sdfx_status sdfx_server_wait(sdfx_server* server, uint64_t ticket, double* latency_ms);
+
+/* Cancels a request that is still queued: *cancelled_out = 1 and its wait
+ * returns SDFX_ERR_CANCELLED. A request already running or done is left as it
+ * is (*cancelled_out = 0). */
+sdfx_status sdfx_server_cancel(sdfx_server* server, uint64_t ticket, int* cancelled_out);
 
 /* *done_out = 1 when the request is done (it must still be waited for). */
 sdfx_status sdfx_server_poll(sdfx_server* server, uint64_t ticket, int* done_out);

@@ -27,6 +27,7 @@ ERR_DEVICE = -8
 ERR_OUT_OF_MEMORY = -9
 ERR_INVALID_GRID = -10
 ERR_OVER_BUDGET = -11
+ERR_CANCELLED = -12
 
 _f = ctypes.POINTER(ctypes.c_float)
 _u32 = ctypes.POINTER(ctypes.c_uint32)
@@ -81,6 +82,7 @@ SIGNATURES = {
     "sdfx_server_submit_points": (_i, [_vp, ctypes.c_uint32, _vp, _u64v, _i, _i, _vp, _u64]),
     "sdfx_server_submit_grid": (_i, [_vp, ctypes.c_uint32, _f, _f, _u64, _i, _vp, _u64]),
     "sdfx_server_wait": (_i, [_vp, _u64v, ctypes.POINTER(ctypes.c_double)]),
+    "sdfx_server_cancel": (_i, [_vp, _u64v, ctypes.POINTER(_i)]),
     "sdfx_server_poll": (_i, [_vp, _u64v, ctypes.POINTER(_i)]),
     "sdfx_server_get_stats": (_i, [_vp, ctypes.POINTER(ServerStats)]),
     "sdfx_profile_enable": (None, [_i]),

@@ -581,6 +581,31 @@ sdfx_status sdfx_server_wait(sdfx_server* sv, uint64_t ticket, double* latency_m
     });
 }
 
+sdfx_status sdfx_server_cancel(sdfx_server* sv, uint64_t ticket, int* cancelled_out) {
+    return server_guarded([&]() -> sdfx_status {
+        if (!sv || !cancelled_out) server_fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: server or cancelled_out");
+        *cancelled_out = 0;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            auto it = sv->tickets.find(ticket);
+            if (it == sv->tickets.end())
+                server_fail(SDFX_ERR_INVALID_ARGUMENT, "unknown ticket " + std::to_string(ticket));
+            const std::shared_ptr<ServerRequest> r = it->second;
+            auto q = std::find(sv->queue.begin(), sv->queue.end(), r);
+            if (q == sv->queue.end()) return SDFX_OK;  // running or done
+            sv->queue.erase(q);
+            r->status = SDFX_ERR_CANCELLED;
+            r->message = "request cancelled";
+            r->t_done = Clock::now();
+            r->done = true;
+            *cancelled_out = 1;
+        }
+        sv->cv_done.notify_all();
+        sv->cv_work.notify_all();  // the queue's front may have changed
+        return SDFX_OK;
+    });
+}
+
 sdfx_status sdfx_server_poll(sdfx_server* sv, uint64_t ticket, int* done_out) {
     return server_guarded([&]() -> sdfx_status {
         if (!sv || !done_out) server_fail(SDFX_ERR_INVALID_ARGUMENT, "null pointer: server or done_out");

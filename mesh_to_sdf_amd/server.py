@@ -137,6 +137,13 @@ class SdfServer:
             raise SdfxError(st, _lib.lib().sdfx_last_error().decode())
         return (None if st == _lib.EMPTY_RESULT else ticket.out), ms.value
 
+    def cancel(self, ticket: Ticket) -> bool:
+        """Cancels the request if it is still queued (its wait then raises
+        ERR_CANCELLED); False when it already runs or is done."""
+        c = ctypes.c_int(0)
+        _lib.check(_lib.lib().sdfx_server_cancel(self._h, ticket.ticket, ctypes.byref(c)))
+        return bool(c.value)
+
     def done(self, ticket: Ticket) -> bool:
         d = ctypes.c_int(0)
         _lib.check(_lib.lib().sdfx_server_poll(self._h, ticket.ticket, ctypes.byref(d)))

@@ -170,3 +170,34 @@ def test_batched_mesh_admission(sdf):
             many.add_meshes(descs)
         assert e.value.status == _lib.ERR_INDEX_OUT_OF_RANGE and "mesh 1" in e.value.message
         assert many.stats()["meshes"] == len(meshes) + 2  # the two good ones of the batch were added
+
+
+def test_cancel_queued_request(sdf):
+    """A request still queued behind a busy worker can be cancelled; the others
+    are unaffected, and a finished request cannot be."""
+    from mesh_to_sdf_amd import _lib
+    v, f = S.torus(1.0, 0.35, 128, 64, 0.002, seed=3)
+    small = S.icosphere(2)
+    with _server(workers=1) as srv:
+        big = srv.add_mesh(v, sdf.Topology.TriangleList(f))
+        sm = srv.add_mesh(small[0], sdf.Topology.TriangleList(small[1]))
+        first = srv.submit_grid(big, sdf.Grid(*S.grid_for(v, (224, 224, 224))), 0)
+        qs = [S.uniform_queries(small[0], 700, seed=50 + k) for k in range(3)]
+        tks = [srv.submit_points(sm, q, sdf.AccelerationMethod.RtreeBvh) for q in qs]
+        assert srv.cancel(tks[1])
+        srv.wait(first)
+        for k in (0, 2):
+            got, _ = srv.wait(tks[k])
+            alone = sdf.generate_sdf(small[0], sdf.Topology.TriangleList(small[1]), qs[k],
+                                     sdf.AccelerationMethod.RtreeBvh)
+            assert np.array_equal(bits(got.cpu().numpy()), bits(alone))
+        with pytest.raises(_lib.SdfxError) as e:
+            srv.wait(tks[1])
+        assert e.value.status == _lib.ERR_CANCELLED
+        t = srv.submit_points(sm, qs[0], sdf.AccelerationMethod.RtreeBvh)
+        srv.wait(t)  # waited tickets are gone; a done, not yet waited one cannot be cancelled
+        t2 = srv.submit_points(sm, qs[0], sdf.AccelerationMethod.RtreeBvh)
+        while not srv.done(t2):
+            pass
+        assert not srv.cancel(t2)
+        srv.wait(t2)
