@@ -38,6 +38,8 @@ sys.path.insert(0, ROOT)
 WORKLOADS = {
     "grid": "generate_grid_sdf Raycast, jittered torus 65,536 tris, 128^3 cells",
     "rtree_bvh": "generate_sdf RtreeBvh, icosphere L3 1,280 tris, 10,000 uniform queries",
+    "none": "generate_sdf None(Raycast), icosphere L3 1,280 tris, 1,048,576 uniform queries",
+    "rtree": "generate_sdf Rtree, noise sphere L7 327,680 tris, 4,194,304 queries",
     "bvh": "generate_sdf Bvh(Raycast), noise sphere L7 327,680 tris, 4,194,304 queries",
     "grid_batch": "batched generate_grid_sdf Raycast, 256 jittered icospheres/tori (1,280-327,680 tris, "
                   "log-uniform), one grid each with axes log-uniform in [32, 128]",
@@ -201,9 +203,13 @@ def main():
         if args.workload == "rtree_bvh":
             v, f, q = S.config1()
             method = (3, 0)
+        elif args.workload == "none":
+            v, f = S.icosphere(3)
+            q = S.uniform_queries(v, 1 << 20, S.SEED_BASE + 11)
+            method = (0, 0)
         else:
             v, f, q = S.config3()
-            method = (1, 0)
+            method = (2, 1) if args.workload == "rtree" else (1, 0)
         dv = torch.from_numpy(v).to(dev)
         df = torch.from_numpy(f.reshape(-1).astype(np.int32)).to(dev)
         dq = torch.from_numpy(q).to(dev)
@@ -308,6 +314,18 @@ def main():
                 "work_per_launch": f"{step_bytes / launches_per_step:.0f} B ({per_sample:.0f} B x {samples} samples "
                                    f"+ {mesh_bytes} B of mesh, over {launches_per_step:g} launches per call)"}
     valu = None
+    # FP32 VALU ceiling of this code: without FMA contraction (the reference's
+    # rounding) and without packed FP32, one lane retires one FP32 operation
+    # per instruction, a quarter of the 157.3 TFLOP/s peak (which counts an FMA
+    # as two and packed pairs as two).
+    valu_ceiling = 157.3 / 4
+    if args.workload == "none" and dom_name.startswith("k_none_raycast"):
+        # 116 FP32 operations per (query, triangle) pair: closest point and
+        # distance (80) and the +X ray test (36), geo.rs as written
+        pairs = float(len(q)) * len(f)
+        tflops = 116.0 * pairs / launches_per_step / avg_launch_s / 1e12
+        valu = {"achieved": tflops, "peak": 157.3, "unit": "TFLOP/s", "frac": tflops / 157.3,
+                "frac_of_unfused_ceiling": tflops / valu_ceiling, "pairs_per_launch": pairs / launches_per_step}
     if args.workload.startswith("grid"):
         # diagnostic beside the headline: FP32 VALU work of the distance evaluations
         # (80 operations per point-triangle distance, geo.rs:26-30, 70-138 as written)
@@ -319,7 +337,7 @@ def main():
         if evals:
             tflops = 80.0 * evals / dom_launches / avg_launch_s / 1e12
             valu = {"achieved": tflops, "peak": 157.3, "unit": "TFLOP/s", "frac": tflops / 157.3,
-                    "evals_per_launch": evals / dom_launches}
+                    "frac_of_unfused_ceiling": tflops / valu_ceiling, "evals_per_launch": evals / dom_launches}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, v, f, locals())
 
@@ -590,9 +608,18 @@ def cpu_baseline(workload, v, f, env):
         return {"value": len(q) / t, "unit": "samples/s", "cores": 1, "kind": "port",
                 "sample": "full workload; single-threaded like generic/rtree_bvh.rs:150-152", "seconds": t,
                 "speedup": None}
+    if workload == "none":
+        q = env["q"][:65_536]
+        t0 = time.perf_counter()
+        O.generate_sdf(v, f, O.TOPO_LIST, q, O.ACCEL_NONE, O.SIGN_RAYCAST, threads)
+        t = time.perf_counter() - t0
+        return {"value": len(q) / t, "unit": "samples/s", "cores": threads, "kind": "port",
+                "sample": f"first 65,536 of the 1,048,576 queries, {threads} threads (default.rs par_iter)",
+                "seconds": t, "speedup": None}
+    accel = O.ACCEL_RTREE if workload == "rtree" else O.ACCEL_BVH
     q = env["q"][:200_000]
     t0 = time.perf_counter()
-    O.generate_sdf(v, f, O.TOPO_LIST, q, O.ACCEL_BVH, O.SIGN_RAYCAST, threads)
+    O.generate_sdf(v, f, O.TOPO_LIST, q, accel, O.SIGN_NORMAL if workload == "rtree" else O.SIGN_RAYCAST, threads)
     t = time.perf_counter() - t0
     return {"value": len(q) / t, "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"first 200,000 of the 4,194,304 queries, {threads} threads (rayon par_iter in the reference)",
