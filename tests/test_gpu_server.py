@@ -201,3 +201,42 @@ def test_cancel_queued_request(sdf):
             pass
         assert not srv.cancel(t2)
         srv.wait(t2)
+
+
+def test_concurrent_clients(sdf):
+    """Several client threads submit and wait at once (the server's calls are
+    thread-safe); every answer is the one the plain entry points give."""
+    import threading
+    meshes = _meshes()
+    with _server(workers=3) as srv:
+        ids = srv.add_meshes([(v, sdf.Topology.TriangleList(f)) for v, f in meshes])
+        errors = []
+
+        def client(c):
+            try:
+                g = np.random.default_rng(900 + c)
+                for k in range(6):
+                    mi = int(g.integers(len(meshes)))
+                    v, f = meshes[mi]
+                    if k % 2:
+                        grid = sdf.Grid(*S.grid_for(v, tuple(int(x) for x in g.integers(8, 40, size=3))))
+                        got, _ = srv.wait(srv.submit_grid(ids[mi], grid, 0))
+                        ref = sdf.generate_grid_sdf(v, sdf.Topology.TriangleList(f), grid, 0)
+                    else:
+                        q = S.uniform_queries(v, int(g.integers(100, 5000)), seed=1000 * c + k)
+                        got, _ = srv.wait(srv.submit_points(ids[mi], q, sdf.AccelerationMethod.RtreeBvh))
+                        ref = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, sdf.AccelerationMethod.RtreeBvh)
+                    if not np.array_equal(bits(got.cpu().numpy()), bits(ref)):
+                        errors.append((c, k))
+            except Exception as e:  # reported below; a thread cannot fail the test itself
+                errors.append((c, repr(e)))
+
+        threads = [threading.Thread(target=client, args=(c,)) for c in range(6)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=100)
+        assert not any(t.is_alive() for t in threads)
+        assert not errors, errors
+        st = srv.stats()
+        assert st["requests_done"] == 36 and st["queued"] == 0 and st["inflight_bytes"] == 0
