@@ -526,34 +526,58 @@ __device__ __forceinline__ void visit_leaf_nearest(const TriRec* __restrict__ le
     }
 }
 
-// Minimum d^2 over all triangles, closest child first.
-__device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
-                               float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
-    Nearest best{INFINITY, 0xFFFFFFFFu, -1};
-    if (root < 0) {
-        visit_leaf_nearest(leaves, ~root, px, py, pz, best, ts);
-        return best;
+// Nearest search of one lane from `start` (a node, or ~leaf), closest child
+// first, folding into the lane's running best. With SHARED, nodes are pruned
+// against min(best, *shared), a word in LDS that the lanes of a wave lower
+// together (atomicMin of the d^2 bits: d^2 >= 0, so bit order is float order).
+// Returns false when more than `budget` nodes were visited with nodes left
+// (the search is incomplete; best holds what was found so far) if budget is
+// below `cap`; more than `cap` visits means a malformed tree (overflow).
+//
+// Pruning keeps a box when box_d2 * PRUNE_SLACK <= bound, and bound is always
+// the d^2 of a triangle already seen, so a box is dropped only if every
+// triangle in it is strictly farther than one already seen. The answer is
+// therefore the minimum of (d^2, topology index) over all triangles whatever
+// the visiting order, start nodes or initial bound: the searches below that
+// split one query's tree among lanes return the same answer as one lane.
+template <bool SHARED>
+__device__ bool nearest_from(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t start,
+                             float px, float py, float pz, uint32_t budget, uint32_t cap, Nearest& best,
+                             uint32_t* shared, bool& overflow, TravStats& ts) {
+    auto bound = [&]() {
+        return SHARED ? fminf(best.d2, bitsf(*(volatile uint32_t*)shared)) : best.d2;
+    };
+    auto leaf = [&](int32_t lf) {
+        const float before = best.d2;
+        visit_leaf_nearest(leaves, lf, px, py, pz, best, ts);
+        if (SHARED && best.d2 < before) atomicMin(shared, fbits(best.d2));
+    };
+    if (start < 0) {
+        leaf(~start);
+        return true;
     }
     LaneStack st;
-    int32_t node = root;
+    int32_t node = start;
     for (uint32_t visits = 0;; ++visits) {
-        if (visits > max_visits) {  // a tree visits each node once; more means a malformed tree
-            overflow = true;
+        if (visits > budget) {
+            if (budget < cap) return false;
+            overflow = true;  // a tree visits each node once; more means a malformed tree
             break;
         }
         ts.nodes++;
         const BvhNode nd = nodes[node];
         float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
         float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
-        bool gl = dl * PRUNE_SLACK <= best.d2;
-        bool gr = dr * PRUNE_SLACK <= best.d2;
+        float b = bound();
+        bool gl = dl * PRUNE_SLACK <= b;
+        bool gr = dr * PRUNE_SLACK <= b;
         if (gl && nd.left < 0) {
-            visit_leaf_nearest(leaves, ~nd.left, px, py, pz, best, ts);
+            leaf(~nd.left);
             gl = false;
-            gr = dr * PRUNE_SLACK <= best.d2;
+            gr = dr * PRUNE_SLACK <= bound();
         }
         if (gr && nd.right < 0) {
-            visit_leaf_nearest(leaves, ~nd.right, px, py, pz, best, ts);
+            leaf(~nd.right);
             gr = false;
         }
         if (gl && gr) {
@@ -572,9 +596,9 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
         }
         bool found = false;
         while (!st.empty()) {
-            uint32_t a, b;
-            st.pop(a, b);
-            if (bitsf(b) * PRUNE_SLACK <= best.d2) {
+            uint32_t a, bb;
+            st.pop(a, bb);
+            if (bitsf(bb) * PRUNE_SLACK <= bound()) {
                 node = (int32_t)a;
                 found = true;
                 break;
@@ -582,6 +606,14 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
         }
         if (!found) break;
     }
+    return true;
+}
+
+// Minimum d^2 over all triangles, closest child first.
+__device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
+                               float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
+    Nearest best{INFINITY, 0xFFFFFFFFu, -1};
+    nearest_from<false>(nodes, leaves, root, px, py, pz, max_visits, max_visits, best, nullptr, overflow, ts);
     return best;
 }
 
@@ -721,29 +753,57 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
     return insides > 1;
 }
 
-// The answer of one query point in mode MODE (the per-query work of every
-// query kernel below).
+// The answer of a nearest-search mode from the nearest triangle (d^2, leaf)
+// and the 3-ray vote.
 template <int MODE>
-__device__ __forceinline__ void query_point(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves,
+__device__ __forceinline__ float nearest_answer(const Nearest& nb, bool inside, float px, float py, float pz,
+                                                const TriRec* __restrict__ leaves) {
+    if (MODE == QM_NEAREST_SIGNED) return nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
+    float d = sqrtf(nb.d2);
+    if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
+    return inside ? -d : d;
+}
+
+// A query whose nearest search passed the visit budget: its index, the best
+// triangle found so far (a bound for the rest of the search) and its vote.
+struct DeferRec {
+    uint64_t q;
+    uint32_t d2;  // bits of the best d^2 so far
+    uint32_t index;
+    int32_t leaf;
+    uint32_t inside;
+};
+
+// The answer of one query point in mode MODE (the per-query work of every
+// query kernel below). With `defer`, a nearest search that passes `budget`
+// visits stops and fills *defer instead of out[i] (returns true); the vote
+// still runs here.
+template <int MODE>
+__device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves,
                                             int32_t root, uint32_t max_visits, const float* __restrict__ q,
-                                            uint64_t i, float* __restrict__ out, DevFlags* flags, TravStats& ts) {
+                                            uint64_t i, float* __restrict__ out, DevFlags* flags, TravStats& ts,
+                                            uint32_t budget = 0xFFFFFFFFu, DeferRec* defer = nullptr) {
     const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
-    bool overflow = false;
+    bool overflow = false, deferred = false;
     if (MODE == QM_NORMAL_FOLD) {
         bool nan_seen = false;
         out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
         if (nan_seen) atomicOr(&flags->nan_distance, 1u);
     } else {
-        Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
-        if (MODE == QM_NEAREST_SIGNED) {
-            out[i] = nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
+        Nearest nb{INFINITY, 0xFFFFFFFFu, -1};
+        const bool done = nearest_from<false>(nodes, leaves, root, px, py, pz, defer ? min(budget, max_visits) : max_visits,
+                                              max_visits, nb, nullptr, overflow, ts);
+        const bool inside = MODE != QM_NEAREST_SIGNED &&
+                            bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
+        if (done) {
+            out[i] = nearest_answer<MODE>(nb, inside, px, py, pz, leaves);
         } else {
-            float d = sqrtf(nb.d2);
-            if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
-            out[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? -d : d;
+            *defer = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u};
+            deferred = true;
         }
     }
     if (overflow) atomicOr(&flags->internal, 1u);
+    return deferred;
 }
 
 template <int MODE>
@@ -751,14 +811,29 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
                                                     const TriRec* __restrict__ leaves, int32_t root,
                                                     uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                     const uint32_t* __restrict__ order, float* __restrict__ out,
-                                                    DevFlags* flags, unsigned long long* stats) {
+                                                    DevFlags* flags, unsigned long long* stats, uint32_t budget,
+                                                    DeferRec* __restrict__ deferred, uint32_t* n_deferred) {
     // with a Morton order, each XCD walks one contiguous stretch of it
     uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = i < nq;  // no early return: the statistics below use block barriers
     TravStats ts;
+    bool dfr = false;
+    DeferRec rec;
     if (active) {
         if (order) i = order[i];  // lanes of a wave take Morton-adjacent queries
-        query_point<MODE>(nodes, leaves, root, max_visits, q, i, out, flags, ts);
+        dfr = query_point<MODE>(nodes, leaves, root, max_visits, q, i, out, flags, ts, budget,
+                                deferred ? &rec : nullptr);
+    }
+    if (deferred) {  // kernel argument: uniform; one atomic per wave appends its deferred queries
+        const uint64_t m = __ballot(dfr);
+        if (m) {
+            const uint32_t lane = threadIdx.x & 63u;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(n_deferred, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (dfr) deferred[base + __popcll(m & ((1ull << lane) - 1ull))] = rec;
+        }
     }
     if (stats) {  // kernel argument: uniform, every lane of the block gets here
         const uint64_t nb = gridDim.x;
@@ -775,6 +850,137 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
         }
         block_count(stats + 4 * nb, (threadIdx.x & 63u) == 0 ? 64u * mx : 0u);
         block_count(stats + 5 * nb, (threadIdx.x & 63u) == 0 ? 64u * mv : 0u);
+    }
+}
+
+// Entries per wave in the work pool of k_lbvh_coop (node, box d^2 bits).
+constexpr uint32_t COOP_POOL = 128;
+
+// The deferred nearest searches, one wave per query, wave-synchronous: each
+// step every busy lane visits one node (closest child first, the farther
+// child kept on its own stack), idle lanes take farther children that busy
+// lanes donate to a pool in LDS while the pool holds fewer than 64 entries,
+// and nodes are pruned against the wave's minimum d^2. The answer is the
+// wave's minimum of (d^2, topology index), starting from the best triangle
+// the first pass found (see nearest_from for why the order does not matter).
+// Persistent waves take queries from *next until all *n_deferred are done.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_lbvh_coop(const BvhNode* __restrict__ nodes,
+                                                   const TriRec* __restrict__ leaves, int32_t root,
+                                                   uint32_t max_visits, const float* __restrict__ q,
+                                                   const DeferRec* __restrict__ recs, const uint32_t* n_deferred,
+                                                   uint32_t* next, float* __restrict__ out, DevFlags* flags,
+                                                   unsigned long long* stats) {
+    __shared__ uint32_t pool_node[4][COOP_POOL], pool_d2[4][COOP_POOL];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t count = *n_deferred;
+    bool overflow = false;
+    TravStats ts;
+    for (;;) {
+        uint32_t j = 0;
+        if (lane == 0) j = atomicAdd(next, 1u);
+        j = __shfl(j, 0, 64);
+        if (j >= count) break;
+        const DeferRec r = recs[j];
+        const float px = q[3 * r.q], py = q[3 * r.q + 1], pz = q[3 * r.q + 2];
+        Nearest best = lane == 0 ? Nearest{bitsf(r.d2), r.index, r.leaf} : Nearest{INFINITY, 0xFFFFFFFFu, -1};
+        LaneStack st;
+        int32_t cur = -1;  // the lane's node, -1 when idle (its stack is then empty)
+        uint32_t pn = 1;   // pool entries: the same value in every lane
+        if (lane == 0) {
+            pool_node[w][0] = (uint32_t)root;
+            pool_d2[w][0] = 0u;
+        }
+        uint32_t visits = 0;
+        for (;;) {
+            float bound = best.d2;
+            for (int off = 32; off > 0; off >>= 1) bound = fminf(bound, __shfl_xor(bound, off, 64));
+            // idle lanes take the newest pool entries
+            const uint64_t idle = __ballot(cur < 0);
+            if (cur < 0) {
+                const uint32_t rk = (uint32_t)__popcll(idle & below);
+                if (rk < pn) {
+                    const uint32_t slot = pn - 1u - rk;
+                    if (bitsf(pool_d2[w][slot]) * PRUNE_SLACK <= bound) cur = (int32_t)pool_node[w][slot];
+                }
+            }
+            pn -= min(pn, (uint32_t)__popcll(idle));
+            if (__ballot(cur >= 0) == 0 && pn == 0) break;
+            bool has_far = false;
+            uint32_t far_node = 0, far_d2 = 0;
+            if (cur >= 0) {
+                if (++visits > max_visits) {  // a tree visits each node once; more means a malformed tree
+                    overflow = true;
+                    cur = -1;
+                } else {
+                    ts.nodes++;
+                    const BvhNode nd = nodes[cur];
+                    const float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
+                    const float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
+                    bool gl = dl * PRUNE_SLACK <= fminf(bound, best.d2);
+                    bool gr = dr * PRUNE_SLACK <= fminf(bound, best.d2);
+                    if (gl && nd.left < 0) {
+                        visit_leaf_nearest(leaves, ~nd.left, px, py, pz, best, ts);
+                        gl = false;
+                        gr = dr * PRUNE_SLACK <= fminf(bound, best.d2);
+                    }
+                    if (gr && nd.right < 0) {
+                        visit_leaf_nearest(leaves, ~nd.right, px, py, pz, best, ts);
+                        gr = false;
+                    }
+                    if (gl && gr) {
+                        const bool left_first = dl <= dr;
+                        cur = left_first ? nd.left : nd.right;
+                        far_node = (uint32_t)(left_first ? nd.right : nd.left);
+                        far_d2 = fbits(left_first ? dr : dl);
+                        has_far = true;
+                    } else if (gl) {
+                        cur = nd.left;
+                    } else if (gr) {
+                        cur = nd.right;
+                    } else {
+                        cur = -1;
+                        while (!st.empty()) {
+                            uint32_t a, bb;
+                            st.pop(a, bb);
+                            if (bitsf(bb) * PRUNE_SLACK <= fminf(bound, best.d2)) {
+                                cur = (int32_t)a;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+            // farther children go to the pool while it runs low (at most
+            // 63 + 64 < COOP_POOL entries), else to the lane's own stack
+            const uint64_t donors = __ballot(has_far);
+            if (has_far) {
+                if (pn < 64u) {
+                    const uint32_t slot = pn + (uint32_t)__popcll(donors & below);
+                    pool_node[w][slot] = far_node;
+                    pool_d2[w][slot] = far_d2;
+                } else if (!st.push(far_node, far_d2)) {
+                    overflow = true;
+                }
+            }
+            if (pn < 64u) pn += (uint32_t)__popcll(donors);
+        }
+        // lexicographic minimum of (d^2, index) over the wave, and its leaf
+        const uint64_t key = ((uint64_t)fbits(best.d2) << 32) | best.index;
+        uint64_t mk = key;
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = __shfl_xor(mk, off, 64);
+            mk = o < mk ? o : mk;
+        }
+        const int wl = __ffsll((unsigned long long)__ballot(key == mk)) - 1;
+        Nearest nb{bitsf((uint32_t)(mk >> 32)), (uint32_t)mk, __shfl(best.leaf, wl, 64)};
+        if (lane == 0) out[r.q] = nearest_answer<MODE>(nb, r.inside != 0u, px, py, pz, leaves);
+    }
+    if (overflow) atomicOr(&flags->internal, 1u);
+    if (stats) {
+        block_count(stats, ts.nodes);
+        block_count(stats + gridDim.x, ts.leaves);
     }
 }
 
@@ -812,34 +1018,77 @@ static const uint32_t* query_order(const Lbvh& bvh, const float* d_q, uint64_t n
     return vals;
 }
 
+template <int MODE>
+static void launch_query(const Lbvh& bvh, const float* d_q, uint64_t nq, const uint32_t* order, unsigned blocks,
+                         float* d_out, DevFlags* d_flags, unsigned long long* stats, uint32_t budget,
+                         DeferRec* recs, uint32_t* n_deferred, hipStream_t s) {
+    SDFX_LAUNCH(k_lbvh_query<MODE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(), bvh.leaves.get(), bvh.root,
+                visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats, budget, recs, n_deferred);
+}
+
+template <int MODE>
+static void launch_coop(const Lbvh& bvh, uint32_t* words, const float* d_q, const DeferRec* recs, unsigned blocks,
+                        float* d_out, DevFlags* d_flags, unsigned long long* stats, hipStream_t s) {
+    // words: n_deferred, next
+    SDFX_LAUNCH(k_lbvh_coop<MODE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(), bvh.leaves.get(), bvh.root,
+                visit_cap(bvh), d_q, recs, words, words + 1, d_out, d_flags, stats);
+}
+
 void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, float* d_out, DevFlags* d_flags,
                 Arena& A) {
     if (!nq || !bvh.n) return;
     hipStream_t s = A.stream;
     const uint32_t* order = query_order(bvh, d_q, nq, A);
     const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
-    unsigned long long* stats = nullptr;
-    if (profile_enabled() && mode != QM_NORMAL_FOLD) {
-        stats = A.alloc<unsigned long long>(6ull * blocks);
-        fill_u32(reinterpret_cast<uint32_t*>(stats), 12ull * blocks, 0u, s);
+    // Nearest searches longer than the visit budget finish in a second pass
+    // that gives each of them a whole wave: a few interior queries can need
+    // tens of thousands of visits, and one lane walking them alone would set
+    // the pass's duration.
+    const uint64_t budget = (uint64_t)std::max<int64_t>(tuning().lbvh_visit_budget, 1);
+    const bool split = mode != QM_NORMAL_FOLD && bvh.root >= 0 && budget < visit_cap(bvh);
+    DeferRec* recs = nullptr;
+    uint32_t* words = nullptr;  // n_deferred, next
+    if (split) {
+        recs = A.alloc<DeferRec>(nq);
+        words = A.alloc<uint32_t>(2);
+        fill_u32(words, 2, 0u, s);
     }
+    unsigned long long* stats = nullptr;
+    const unsigned coop_blocks = (unsigned)std::min<uint64_t>(2048, (nq + 3) / 4);
+    if (profile_enabled() && mode != QM_NORMAL_FOLD) {
+        stats = A.alloc<unsigned long long>(6ull * blocks + 2ull * coop_blocks);
+        fill_u32(reinterpret_cast<uint32_t*>(stats), 12ull * blocks + 4ull * coop_blocks, 0u, s);
+    }
+    const uint32_t b32 = split ? (uint32_t)budget : 0xFFFFFFFFu;
     switch (mode) {
         case QM_RAYCAST_VOTE:
-            SDFX_LAUNCH(k_lbvh_query<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
+            launch_query<QM_RAYCAST_VOTE>(bvh, d_q, nq, order, blocks, d_out, d_flags, stats, b32, recs, words, s);
             break;
         case QM_NORMAL_FOLD:
-            SDFX_LAUNCH(k_lbvh_query<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
+            launch_query<QM_NORMAL_FOLD>(bvh, d_q, nq, order, blocks, d_out, d_flags, stats, b32, nullptr, nullptr, s);
             break;
         case QM_NEAREST_SIGNED:
-            SDFX_LAUNCH(k_lbvh_query<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
+            launch_query<QM_NEAREST_SIGNED>(bvh, d_q, nq, order, blocks, d_out, d_flags, stats, b32, recs, words, s);
             break;
         case QM_NEAREST_VOTE:
-            SDFX_LAUNCH(k_lbvh_query<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                               bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats);
+            launch_query<QM_NEAREST_VOTE>(bvh, d_q, nq, order, blocks, d_out, d_flags, stats, b32, recs, words, s);
             break;
+    }
+    unsigned long long* cstats = stats ? stats + 6ull * blocks : nullptr;
+    if (split) {
+        switch (mode) {
+            case QM_RAYCAST_VOTE:
+                launch_coop<QM_RAYCAST_VOTE>(bvh, words, d_q, recs, coop_blocks, d_out, d_flags, cstats, s);
+                break;
+            case QM_NEAREST_SIGNED:
+                launch_coop<QM_NEAREST_SIGNED>(bvh, words, d_q, recs, coop_blocks, d_out, d_flags, cstats, s);
+                break;
+            case QM_NEAREST_VOTE:
+                launch_coop<QM_NEAREST_VOTE>(bvh, words, d_q, recs, coop_blocks, d_out, d_flags, cstats, s);
+                break;
+            default:
+                break;
+        }
     }
     SDFX_HIP(hipGetLastError());
     if (stats) {
@@ -850,12 +1099,17 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         profile_count("lbvh.nearest_wave_slots", sum_block_counters(stats + 4ull * blocks, blocks, s));
         profile_count("lbvh.vote_wave_slots", sum_block_counters(stats + 5ull * blocks, blocks, s));
         profile_count("lbvh.queries", nq);
+        if (split) {
+            profile_count("lbvh.deferred_nodes", sum_block_counters(cstats, coop_blocks, s));
+            profile_count("lbvh.deferred_leaves", sum_block_counters(cstats + coop_blocks, coop_blocks, s));
+            uint32_t nd = 0;
+            SDFX_HIP(hipMemcpyAsync(&nd, words, sizeof(nd), hipMemcpyDeviceToHost, s));
+            SDFX_HIP(hipStreamSynchronize(s));
+            profile_count("lbvh.deferred_queries", nd);
+        }
     }
 }
 
-// ---------------------------------------------------------------- batched queries
-
-// One request of a batched point query as the kernels see it.
 struct PointReq {
     const BvhNode* nodes;
     const TriRec* leaves;

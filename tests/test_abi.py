@@ -59,3 +59,20 @@ def test_no_oracle_in_product():
             if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")) or f == "Makefile":
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "liboracle" not in text and "sdf_oracle" not in text and "import oracle" not in text, f
+
+
+def test_tuning_knobs_and_ranges():
+    """Every knob reads back what was set, out-of-range values and unknown
+    keys are refused (no GPU needed: the knobs are host state)."""
+    from mesh_to_sdf_amd import _lib
+    L = _lib.lib()
+    for key, good, bad in (("exhaustive_max_pairs", 12345, None), ("grid_prop_blocks", 512, 4),
+                           ("lbvh_visit_budget", 64, 15)):
+        old = L.sdfx_get_tuning(key.encode())
+        assert old >= 0 or key == "exhaustive_max_pairs"
+        assert L.sdfx_set_tuning(key.encode(), good) == 0 and L.sdfx_get_tuning(key.encode()) == good
+        if bad is not None:
+            assert L.sdfx_set_tuning(key.encode(), bad) == -1 and L.sdfx_get_tuning(key.encode()) == good
+        assert L.sdfx_set_tuning(key.encode(), old) == 0
+    assert L.sdfx_set_tuning(b"lbvh_visit_budget", 1 << 31) == -1
+    assert L.sdfx_set_tuning(b"no_such_knob", 1) == -1 and L.sdfx_get_tuning(b"no_such_knob") == -1

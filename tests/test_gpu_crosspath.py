@@ -89,7 +89,7 @@ def test_device_grid_continuity(sdf):
 def test_results_independent_of_launch_shapes(sdf):
     """Every tuning knob changes launch shapes only: the grid with different
     propagation grid sizes, and the point paths exhaustive or through the
-    LBVH, give the same bits."""
+    LBVH (with any visit budget for the per-query pass), give the same bits."""
     v, f, first, size, count = S.config2()
     grid = sdf.Grid(first, size, (64, 64, 64))
     T = sdf.Topology.TriangleList(f)
@@ -103,6 +103,9 @@ def test_results_independent_of_launch_shapes(sdf):
               sdf.AccelerationMethod.RtreeBvh):
         with sdf._lib.tuned("exhaustive_max_pairs", 0):
             a = sdf.generate_sdf(v, T, q, m)
+            for budget in (16, 100, 2048, (1 << 31) - 1):  # per-wave pass for searches past the budget
+                with sdf._lib.tuned("lbvh_visit_budget", budget):
+                    assert np.array_equal(bits(a), bits(sdf.generate_sdf(v, T, q, m))), (m, budget)
         with sdf._lib.tuned("exhaustive_max_pairs", 1 << 62):
             b = sdf.generate_sdf(v, T, q, m)
         assert np.array_equal(bits(a), bits(b)), m

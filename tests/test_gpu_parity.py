@@ -7,6 +7,8 @@ schedule; against the reference's own heap schedule the signs are bit-exact
 and the distances are measured (the reference's flood fill is itself
 schedule-dependent: its 1-thread and 8-thread runs disagree on ~2% of cells).
 """
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -26,12 +28,24 @@ def assert_bit_exact(gpu, ref, what):
 
 
 METHODS = [(0, 0), (0, 1), (1, 0), (1, 1), (2, 1), (3, 0)]
-PATHS = {"lbvh": 0, "exhaustive": 1 << 62}  # exhaustive_max_pairs forcing each implementation
+# Tuning forcing each implementation: the LBVH, the LBVH with nearly every
+# nearest search finished by the per-wave pass (a 16-visit budget), and the
+# exhaustive kernels.
+PATHS = {"lbvh": {"exhaustive_max_pairs": 0},
+         "lbvh_deferred": {"exhaustive_max_pairs": 0, "lbvh_visit_budget": 16},
+         "exhaustive": {"exhaustive_max_pairs": 1 << 62}}
+
+
+def tuned_all(sdf, knobs):
+    st = contextlib.ExitStack()
+    for key, value in knobs.items():
+        st.enter_context(sdf._lib.tuned(key, value))
+    return st
 
 
 @pytest.fixture(params=sorted(PATHS))
 def path(request, sdf):
-    with sdf._lib.tuned("exhaustive_max_pairs", PATHS[request.param]):
+    with tuned_all(sdf, PATHS[request.param]):
         yield request.param
 
 
@@ -92,13 +106,17 @@ def test_config1_rtree_bvh_full(sdf, oracle, path):
     assert_bit_exact(gpu, ref, "config1")
 
 
-def test_config3_shape_reduced(sdf, oracle):
+@pytest.mark.parametrize("budget", [2048, 256, 16])
+def test_config3_shape_reduced(sdf, oracle, budget):
     """Noise sphere + half near-surface / half uniform queries, Bvh(Raycast), at
-    level 5 (20,480 triangles) and 100k queries (the oracle's BVH keeps it fast)."""
+    level 5 (20,480 triangles) and 100k queries (the oracle's BVH keeps it fast),
+    with the nearest searches longer than `budget` visits finished per wave."""
     v, f, q = S.config3(n_queries=100_000, level=5)
     s, ref = oracle.generate_sdf(v, f, oracle.TOPO_LIST, q, oracle.ACCEL_BVH, oracle.SIGN_RAYCAST)
-    gpu = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast))
-    assert_bit_exact(gpu, ref, "config3 reduced")
+    with sdf._lib.tuned("lbvh_visit_budget", budget):
+        gpu = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q,
+                               sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast))
+    assert_bit_exact(gpu, ref, f"config3 reduced, budget {budget}")
     diag = S.bbox_diagonal(v)
     d64 = oracle.fp64_distance(v, f, q[:2000])
     err = np.abs(np.abs(gpu[:2000]).astype(np.float64) - d64) / diag
