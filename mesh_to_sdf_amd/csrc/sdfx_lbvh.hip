@@ -767,11 +767,13 @@ __device__ __forceinline__ float nearest_answer(const Nearest& nb, bool inside, 
 // A query whose nearest search passed the visit budget: its index, the best
 // triangle found so far (a bound for the rest of the search) and its vote.
 struct DeferRec {
-    uint64_t q;
-    uint32_t d2;  // bits of the best d^2 so far
+    uint64_t q;    // query index within its request
+    uint32_t d2;   // bits of the best d^2 so far
     uint32_t index;
     int32_t leaf;
     uint32_t inside;
+    uint32_t req;  // request of a batched query (0 otherwise)
+    uint32_t pad;
 };
 
 // The answer of one query point in mode MODE (the per-query work of every
@@ -798,7 +800,7 @@ __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, c
         if (done) {
             out[i] = nearest_answer<MODE>(nb, inside, px, py, pz, leaves);
         } else {
-            *defer = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u};
+            *defer = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u, 0u, 0u};
             deferred = true;
         }
     }
@@ -853,6 +855,19 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
     }
 }
 
+// One request of a batched point query (one tree, its queries and outputs).
+struct PointReq {
+    const BvhNode* nodes;
+    const TriRec* leaves;
+    const float* q;
+    float* out;
+    uint64_t q_off;  // first query of the request in the batch numbering
+    int32_t root;
+    uint32_t max_visits;
+    uint32_t flag;
+    uint32_t pad;
+};
+
 // Entries per wave in the work pool of k_lbvh_coop (node, box d^2 bits).
 constexpr uint32_t COOP_POOL = 128;
 
@@ -865,12 +880,9 @@ constexpr uint32_t COOP_POOL = 128;
 // the first pass found (see nearest_from for why the order does not matter).
 // Persistent waves take queries from *next until all *n_deferred are done.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_lbvh_coop(const BvhNode* __restrict__ nodes,
-                                                   const TriRec* __restrict__ leaves, int32_t root,
-                                                   uint32_t max_visits, const float* __restrict__ q,
+__global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ reqs, PointReq single,
                                                    const DeferRec* __restrict__ recs, const uint32_t* n_deferred,
-                                                   uint32_t* next, float* __restrict__ out, DevFlags* flags,
-                                                   unsigned long long* stats) {
+                                                   uint32_t* next, DevFlags* flags, unsigned long long* stats) {
     __shared__ uint32_t pool_node[4][COOP_POOL], pool_d2[4][COOP_POOL];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -883,13 +895,17 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const BvhNode* __restrict__ n
         j = __shfl(j, 0, 64);
         if (j >= count) break;
         const DeferRec r = recs[j];
-        const float px = q[3 * r.q], py = q[3 * r.q + 1], pz = q[3 * r.q + 2];
+        const PointReq R = reqs ? reqs[r.req] : single;
+        const BvhNode* __restrict__ nodes = R.nodes;
+        const TriRec* __restrict__ leaves = R.leaves;
+        const uint32_t max_visits = R.max_visits;
+        const float px = R.q[3 * r.q], py = R.q[3 * r.q + 1], pz = R.q[3 * r.q + 2];
         Nearest best = lane == 0 ? Nearest{bitsf(r.d2), r.index, r.leaf} : Nearest{INFINITY, 0xFFFFFFFFu, -1};
         LaneStack st;
         int32_t cur = -1;  // the lane's node, -1 when idle (its stack is then empty)
         uint32_t pn = 1;   // pool entries: the same value in every lane
         if (lane == 0) {
-            pool_node[w][0] = (uint32_t)root;
+            pool_node[w][0] = (uint32_t)R.root;
             pool_d2[w][0] = 0u;
         }
         uint32_t visits = 0;
@@ -975,9 +991,10 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const BvhNode* __restrict__ n
         }
         const int wl = __ffsll((unsigned long long)__ballot(key == mk)) - 1;
         Nearest nb{bitsf((uint32_t)(mk >> 32)), (uint32_t)mk, __shfl(best.leaf, wl, 64)};
-        if (lane == 0) out[r.q] = nearest_answer<MODE>(nb, r.inside != 0u, px, py, pz, leaves);
+        if (lane == 0) R.out[r.q] = nearest_answer<MODE>(nb, r.inside != 0u, px, py, pz, leaves);
+        if (__ballot(overflow) && lane == 0) atomicOr(&flags[R.flag].internal, 1u);
+        overflow = false;
     }
-    if (overflow) atomicOr(&flags->internal, 1u);
     if (stats) {
         block_count(stats, ts.nodes);
         block_count(stats + gridDim.x, ts.leaves);
@@ -1026,12 +1043,27 @@ static void launch_query(const Lbvh& bvh, const float* d_q, uint64_t nq, const u
                 visit_cap(bvh), d_q, nq, order, d_out, d_flags, stats, budget, recs, n_deferred);
 }
 
-template <int MODE>
-static void launch_coop(const Lbvh& bvh, uint32_t* words, const float* d_q, const DeferRec* recs, unsigned blocks,
-                        float* d_out, DevFlags* d_flags, unsigned long long* stats, hipStream_t s) {
-    // words: n_deferred, next
-    SDFX_LAUNCH(k_lbvh_coop<MODE>, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(), bvh.leaves.get(), bvh.root,
-                visit_cap(bvh), d_q, recs, words, words + 1, d_out, d_flags, stats);
+// The per-wave pass over the deferred queries: of one tree (reqs null, the
+// tree and buffers in `single`) or of a batch's requests. words: n_deferred, next.
+static void launch_coop(QueryMode mode, const PointReq* reqs, const PointReq& single, uint32_t* words,
+                        const DeferRec* recs, unsigned blocks, DevFlags* d_flags, unsigned long long* stats,
+                        hipStream_t s) {
+    switch (mode) {
+        case QM_RAYCAST_VOTE:
+            SDFX_LAUNCH(k_lbvh_coop<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, reqs, single, recs, words,
+                        words + 1, d_flags, stats);
+            break;
+        case QM_NEAREST_SIGNED:
+            SDFX_LAUNCH(k_lbvh_coop<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, reqs, single, recs, words,
+                        words + 1, d_flags, stats);
+            break;
+        case QM_NEAREST_VOTE:
+            SDFX_LAUNCH(k_lbvh_coop<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, reqs, single, recs, words,
+                        words + 1, d_flags, stats);
+            break;
+        default:
+            break;
+    }
 }
 
 void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, float* d_out, DevFlags* d_flags,
@@ -1076,19 +1108,8 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     }
     unsigned long long* cstats = stats ? stats + 6ull * blocks : nullptr;
     if (split) {
-        switch (mode) {
-            case QM_RAYCAST_VOTE:
-                launch_coop<QM_RAYCAST_VOTE>(bvh, words, d_q, recs, coop_blocks, d_out, d_flags, cstats, s);
-                break;
-            case QM_NEAREST_SIGNED:
-                launch_coop<QM_NEAREST_SIGNED>(bvh, words, d_q, recs, coop_blocks, d_out, d_flags, cstats, s);
-                break;
-            case QM_NEAREST_VOTE:
-                launch_coop<QM_NEAREST_VOTE>(bvh, words, d_q, recs, coop_blocks, d_out, d_flags, cstats, s);
-                break;
-            default:
-                break;
-        }
+        const PointReq single{bvh.nodes.get(), bvh.leaves.get(), d_q, d_out, 0, bvh.root, visit_cap(bvh), 0u, 0u};
+        launch_coop(mode, nullptr, single, words, recs, coop_blocks, d_flags, cstats, s);
     }
     SDFX_HIP(hipGetLastError());
     if (stats) {
@@ -1109,18 +1130,6 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         }
     }
 }
-
-struct PointReq {
-    const BvhNode* nodes;
-    const TriRec* leaves;
-    const float* q;
-    float* out;
-    uint64_t q_off;  // first query of the request in the batch numbering
-    int32_t root;
-    uint32_t max_visits;
-    uint32_t flag;
-    uint32_t pad;
-};
 
 // The request holding batch query g: the last one whose q_off is <= g.
 __device__ __forceinline__ uint32_t point_req_of(const PointReq* __restrict__ reqs, uint32_t nreq, uint64_t g) {
@@ -1165,13 +1174,31 @@ __global__ void __launch_bounds__(256) k_point_morton_batch(const PointReq* __re
 template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
                                                           uint64_t total, const uint32_t* __restrict__ order,
-                                                          DevFlags* flags) {
+                                                          DevFlags* flags, uint32_t budget,
+                                                          DeferRec* __restrict__ deferred, uint32_t* n_deferred) {
     const uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint64_t g = order ? order[i] : i;
-    const PointReq& R = reqs[point_req_of(reqs, nreq, g)];
-    TravStats ts;
-    query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag, ts);
+    bool dfr = false;
+    DeferRec rec;
+    if (i < total) {  // no early return: the wave appends its deferred queries together
+        const uint64_t g = order ? order[i] : i;
+        const uint32_t r = point_req_of(reqs, nreq, g);
+        const PointReq& R = reqs[r];
+        TravStats ts;
+        dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag, ts,
+                                budget, deferred ? &rec : nullptr);
+        rec.req = r;
+    }
+    if (deferred) {  // kernel argument: uniform
+        const uint64_t m = __ballot(dfr);
+        if (m) {
+            const uint32_t lane = threadIdx.x & 63u;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(n_deferred, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (dfr) deferred[base + __popcll(m & ((1ull << lane) - 1ull))] = rec;
+        }
+    }
 }
 
 void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, DevFlags* d_flags, Arena& A) {
@@ -1205,24 +1232,37 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
             order = vals;
         }
         const unsigned blocks = order ? xcd_grid(blocks_for(total, 256)) : blocks_for(total, 256);
+        // long nearest searches finish one per wave, as in lbvh_query
+        const bool split = mode != QM_NORMAL_FOLD;
+        const uint32_t budget = (uint32_t)std::max<int64_t>(tuning().lbvh_visit_budget, 1);
+        DeferRec* recs = nullptr;
+        uint32_t* words = nullptr;  // n_deferred, next
+        if (split) {
+            recs = A.alloc<DeferRec>(total);
+            words = A.alloc<uint32_t>(2);
+            fill_u32(words, 2, 0u, s);
+        }
         switch (mode) {
             case QM_RAYCAST_VOTE:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags);
+                            order, d_flags, budget, recs, words);
                 break;
             case QM_NORMAL_FOLD:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags);
+                            order, d_flags, budget, recs, words);
                 break;
             case QM_NEAREST_SIGNED:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
-                            total, order, d_flags);
+                            total, order, d_flags, budget, recs, words);
                 break;
             case QM_NEAREST_VOTE:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags);
+                            order, d_flags, budget, recs, words);
                 break;
         }
+        if (split)
+            launch_coop(mode, d_reqs, PointReq{}, words, recs, (unsigned)std::min<uint64_t>(2048, (total + 3) / 4),
+                        d_flags, nullptr, s);
         SDFX_HIP(hipGetLastError());
     }
 }

@@ -39,7 +39,7 @@ constexpr uint64_t MESH_BYTES_PER_TRI = 96 + 96 + 64;  // topology TriRecs, LBVH
 constexpr uint64_t GRID_BYTES_PER_CELL = 28;           // keys, flags, frontier items, ray parity bits
 constexpr uint64_t GRID_BYTES_PER_TRI = 40;            // seed boxes, pair counts, owner table
 constexpr uint64_t GRID_TILE_CELLS = 4096;             // each request's cells start on a tile
-constexpr uint64_t POINT_BYTES_PER_QUERY = 24;         // sort keys, order, radix-sort buffers
+constexpr uint64_t POINT_BYTES_PER_QUERY = 56;         // sort keys, order, radix-sort buffers, deferred searches
 constexpr uint64_t REQUEST_BYTES = 4096;               // tables and flags
 // A batch's limits: the grid run's cell capacity and a cap on the table sizes.
 constexpr uint64_t BATCH_MAX_CELLS = 150000000;

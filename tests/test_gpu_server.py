@@ -24,11 +24,14 @@ def _server(budget=4 << 30, workers=2):
     return SdfServer(budget, workers)
 
 
-def test_mixed_requests_equal_alone(sdf):
+@pytest.mark.parametrize("visit_budget", [1024, 16])
+def test_mixed_requests_equal_alone(sdf, visit_budget):
+    """Every method and both grid signs; with a 16-visit budget nearly every
+    nearest search of the batched point pass finishes in the per-wave pass."""
     meshes = _meshes()
     g = np.random.default_rng(11)
     methods = list(sdf.ALL_SDF_ACCELERATION_METHODS)
-    with _server() as srv:
+    with sdf._lib.tuned("lbvh_visit_budget", visit_budget), _server() as srv:
         ids = [srv.add_mesh(v, sdf.Topology.TriangleList(f)) for v, f in meshes]
         jobs = []
         for k in range(36):
