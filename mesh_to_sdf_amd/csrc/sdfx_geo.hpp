@@ -70,6 +70,30 @@ SDFX_HD void make_trirec(float ax, float ay, float az, float bx, float by, float
     t.pad = 0;
 }
 
+// A triangle as the LBVH stores it: the vertices, the degenerate case and the
+// topology index (48 bytes, three 16-byte loads). expand_leaf() rebuilds the
+// full record in registers with make_trirec's expressions (each edge one FP32
+// subtraction, the normal the same cross product), so every value, and every
+// distance computed from them, is bit-identical to the stored TriRec's.
+struct __attribute__((aligned(16))) LeafRec {
+    float ax, ay, az, bx;
+    float by, bz, cx, cy;
+    float cz;
+    uint32_t degen, index, pad;
+};
+static_assert(sizeof(LeafRec) == 48, "LeafRec layout");
+
+SDFX_HD LeafRec make_leafrec(const TriRec& t) {
+    return LeafRec{t.ax, t.ay, t.az, t.bx, t.by, t.bz, t.cx, t.cy, t.cz, t.degen, t.index, 0u};
+}
+
+SDFX_HD TriRec expand_leaf(const LeafRec& l) {
+    TriRec t;
+    make_trirec(l.ax, l.ay, l.az, l.bx, l.by, l.bz, l.cx, l.cy, l.cz, l.index, t);
+    t.degen = l.degen;  // the same value make_trirec derives; kept to skip the comparisons
+    return t;
+}
+
 // Closest point on the triangle to p (geo.rs:70-138), written as the vector
 // p - closest so callers get dist/dist2/signed-direction directly. Vertex
 // regions return p - a etc. (the same values as p.sub(a)).

@@ -65,7 +65,7 @@ struct Lbvh {
     uint64_t n = 0;          // leaves (triangles)
     int32_t root = 0;        // root link (leaf ~0 when n == 1)
     DevBuf<BvhNode> nodes;   // n - 1 internal nodes
-    DevBuf<TriRec> leaves;   // triangles in Morton order (TriRec::index keeps topology order)
+    DevBuf<LeafRec> leaves;  // triangles in Morton order (LeafRec::index keeps topology order)
 };
 
 // Builds the LBVH: centroid bounds, 30-bit Morton codes, stable LDS radix

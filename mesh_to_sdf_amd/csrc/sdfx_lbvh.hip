@@ -43,7 +43,7 @@ __device__ __forceinline__ void tri_centroid(const TriRec& t, float* c) {
 // node) within it; blocks past a small mesh's end exit at once.
 struct BuildSeg {
     const TriRec* tris;  // the mesh's triangles in topology order
-    TriRec* leaves;      // out: n triangles in Morton order
+    LeafRec* leaves;     // out: n triangles in Morton order
     BvhNode* nodes;      // out: n - 1 internal nodes
     uint64_t tri_off;    // first triangle (and sorted position) of the mesh in the batch
     uint64_t node_off;   // first internal node of the mesh in the batch
@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256) k_leaves(const BuildSeg* __restrict__ seg
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.n) return;
     const TriRec t = S.tris[order[S.tri_off + i] - S.tri_off];
-    S.leaves[i] = t;
+    S.leaves[i] = make_leafrec(t);
     Box b;
     tri_bbox(t.ax, t.ay, t.az, t.bx, t.by, t.bz, t.cx, t.cy, t.cz, b);
     g_leaf_box[S.tri_off + i] = b;
@@ -514,10 +514,10 @@ struct TravStats {
     uint32_t nodes = 0, leaves = 0, vote_nodes = 0, vote_leaves = 0;
 };
 
-__device__ __forceinline__ void visit_leaf_nearest(const TriRec* __restrict__ leaves, int32_t leaf, float px,
+__device__ __forceinline__ void visit_leaf_nearest(const LeafRec* __restrict__ leaves, int32_t leaf, float px,
                                                    float py, float pz, Nearest& best, TravStats& ts) {
     ts.leaves++;
-    const TriRec t = leaves[leaf];
+    const TriRec t = expand_leaf(leaves[leaf]);
     float d2 = tri_dist2(px, py, pz, t);
     if (d2 < best.d2 || (d2 == best.d2 && t.index < best.index)) {
         best.d2 = d2;
@@ -541,7 +541,7 @@ __device__ __forceinline__ void visit_leaf_nearest(const TriRec* __restrict__ le
 // the visiting order, start nodes or initial bound: the searches below that
 // split one query's tree among lanes return the same answer as one lane.
 template <bool SHARED>
-__device__ bool nearest_from(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t start,
+__device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t start,
                              float px, float py, float pz, uint32_t budget, uint32_t cap, Nearest& best,
                              uint32_t* shared, bool& overflow, TravStats& ts) {
     auto bound = [&]() {
@@ -571,14 +571,17 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const TriRec* __
         float b = bound();
         bool gl = dl * PRUNE_SLACK <= b;
         bool gr = dr * PRUNE_SLACK <= b;
-        if (gl && nd.left < 0) {
-            leaf(~nd.left);
-            gl = false;
-            gr = dr * PRUNE_SLACK <= bound();
-        }
-        if (gr && nd.right < 0) {
-            leaf(~nd.right);
-            gr = false;
+        // leaf children, left first; one call site, so lanes visiting the
+        // left leaf and lanes visiting the right one share the evaluation
+        const int32_t l0 = gl && nd.left < 0 ? ~nd.left : -1, l1 = gr && nd.right < 0 ? ~nd.right : -1;
+        if (l0 >= 0) gl = false;
+        if (l1 >= 0) gr = false;
+        int32_t pending = l0 >= 0 ? l0 : l1, then = l0 >= 0 ? l1 : -1;
+#pragma unroll 1
+        while (pending >= 0) {
+            leaf(pending);
+            pending = then >= 0 && dr * PRUNE_SLACK <= bound() ? then : -1;  // the right leaf, pruned anew
+            then = -1;
         }
         if (gl && gr) {
             bool left_first = dl <= dr;
@@ -610,7 +613,7 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const TriRec* __
 }
 
 // Minimum d^2 over all triangles, closest child first.
-__device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
+__device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                                float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
     Nearest best{INFINITY, 0xFFFFFFFFu, -1};
     nearest_from<false>(nodes, leaves, root, px, py, pz, max_visits, max_visits, best, nullptr, overflow, ts);
@@ -621,7 +624,7 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const TriRec* 
 // each candidate's signed distance when compare_distances(min_dist, d) is
 // Greater. Nodes are skipped only when their box is farther than any value
 // that could tie with (ulps 2 / eps 1e-6) or beat |min_dist|.
-__device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves,
+__device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves,
                                  int32_t root, float px, float py, float pz, uint32_t max_visits, bool& nan_seen,
                                  bool& overflow) {
     float acc = FLT_MAX;
@@ -630,7 +633,7 @@ __device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec
         return b * b * (1.0f + 1.0f / 1048576.0f);
     };
     auto visit = [&](int32_t leaf) {
-        float d = tri_signed_dist(px, py, pz, leaves[leaf]);
+        float d = tri_signed_dist(px, py, pz, expand_leaf(leaves[leaf]));
         int c = compare_distances(acc, d);
         nan_seen |= c == 2;
         if (c == 1) acc = d;
@@ -689,10 +692,10 @@ __device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const TriRec
     return acc;
 }
 
-__device__ __forceinline__ void ray_leaf(const TriRec* __restrict__ leaves, int32_t leaf, uint32_t axes, float px,
+__device__ __forceinline__ void ray_leaf(const LeafRec* __restrict__ leaves, int32_t leaf, uint32_t axes, float px,
                                          float py, float pz, uint32_t* cnt, TravStats& ts) {
     ts.vote_leaves++;
-    const TriRec t = leaves[leaf];
+    const TriRec t = expand_leaf(leaves[leaf]);
     if ((axes & 1u) && ray_hit<0>(px, py, pz, t, nullptr)) cnt[0]++;
     if ((axes & 2u) && ray_hit<1>(px, py, pz, t, nullptr)) cnt[1]++;
     if ((axes & 4u) && ray_hit<2>(px, py, pz, t, nullptr)) cnt[2]++;
@@ -700,7 +703,7 @@ __device__ __forceinline__ void ray_leaf(const TriRec* __restrict__ leaves, int3
 
 // The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198: the
 // three rays share one traversal; each stack entry carries its live axes.
-__device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves, int32_t root,
+__device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                                 float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
     uint32_t cnt[3] = {0u, 0u, 0u};
     if (root < 0) {
@@ -757,8 +760,9 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const TriRec*
 // and the 3-ray vote.
 template <int MODE>
 __device__ __forceinline__ float nearest_answer(const Nearest& nb, bool inside, float px, float py, float pz,
-                                                const TriRec* __restrict__ leaves) {
-    if (MODE == QM_NEAREST_SIGNED) return nb.leaf >= 0 ? tri_signed_dist(px, py, pz, leaves[nb.leaf]) : sqrtf(nb.d2);
+                                                const LeafRec* __restrict__ leaves) {
+    if (MODE == QM_NEAREST_SIGNED)
+        return nb.leaf >= 0 ? tri_signed_dist(px, py, pz, expand_leaf(leaves[nb.leaf])) : sqrtf(nb.d2);
     float d = sqrtf(nb.d2);
     if (MODE == QM_RAYCAST_VOTE) d = fminf(FLT_MAX, d);  // generic/bvh.rs:131-139 folds from f32::MAX
     return inside ? -d : d;
@@ -781,7 +785,7 @@ struct DeferRec {
 // visits stops and fills *defer instead of out[i] (returns true); the vote
 // still runs here.
 template <int MODE>
-__device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, const TriRec* __restrict__ leaves,
+__device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves,
                                             int32_t root, uint32_t max_visits, const float* __restrict__ q,
                                             uint64_t i, float* __restrict__ out, DevFlags* flags, TravStats& ts,
                                             uint32_t budget = 0xFFFFFFFFu, DeferRec* defer = nullptr) {
@@ -810,7 +814,7 @@ __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, c
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ nodes,
-                                                    const TriRec* __restrict__ leaves, int32_t root,
+                                                    const LeafRec* __restrict__ leaves, int32_t root,
                                                     uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                     const uint32_t* __restrict__ order, float* __restrict__ out,
                                                     DevFlags* flags, unsigned long long* stats, uint32_t budget,
@@ -858,7 +862,7 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
 // One request of a batched point query (one tree, its queries and outputs).
 struct PointReq {
     const BvhNode* nodes;
-    const TriRec* leaves;
+    const LeafRec* leaves;
     const float* q;
     float* out;
     uint64_t q_off;  // first query of the request in the batch numbering
@@ -897,7 +901,7 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
         const DeferRec r = recs[j];
         const PointReq R = reqs ? reqs[r.req] : single;
         const BvhNode* __restrict__ nodes = R.nodes;
-        const TriRec* __restrict__ leaves = R.leaves;
+        const LeafRec* __restrict__ leaves = R.leaves;
         const uint32_t max_visits = R.max_visits;
         const float px = R.q[3 * r.q], py = R.q[3 * r.q + 1], pz = R.q[3 * r.q + 2];
         Nearest best = lane == 0 ? Nearest{bitsf(r.d2), r.index, r.leaf} : Nearest{INFINITY, 0xFFFFFFFFu, -1};
@@ -936,14 +940,15 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
                     const float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
                     bool gl = dl * PRUNE_SLACK <= fminf(bound, best.d2);
                     bool gr = dr * PRUNE_SLACK <= fminf(bound, best.d2);
-                    if (gl && nd.left < 0) {
-                        visit_leaf_nearest(leaves, ~nd.left, px, py, pz, best, ts);
-                        gl = false;
-                        gr = dr * PRUNE_SLACK <= fminf(bound, best.d2);
-                    }
-                    if (gr && nd.right < 0) {
-                        visit_leaf_nearest(leaves, ~nd.right, px, py, pz, best, ts);
-                        gr = false;
+                    const int32_t l0 = gl && nd.left < 0 ? ~nd.left : -1, l1 = gr && nd.right < 0 ? ~nd.right : -1;
+                    if (l0 >= 0) gl = false;
+                    if (l1 >= 0) gr = false;
+                    int32_t pending = l0 >= 0 ? l0 : l1, then = l0 >= 0 ? l1 : -1;
+#pragma unroll 1
+                    while (pending >= 0) {  // one call site for left and right leaves
+                        visit_leaf_nearest(leaves, pending, px, py, pz, best, ts);
+                        pending = then >= 0 && dr * PRUNE_SLACK <= fminf(bound, best.d2) ? then : -1;
+                        then = -1;
                     }
                     if (gl && gr) {
                         const bool left_first = dl <= dr;
@@ -1268,7 +1273,7 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
 }
 
 __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restrict__ nodes,
-                                                         const TriRec* __restrict__ leaves, int32_t root,
+                                                         const LeafRec* __restrict__ leaves, int32_t root,
                                                          uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                          const uint32_t* __restrict__ order, float* __restrict__ vec,
                                                          uint8_t* __restrict__ inside, DevFlags* flags) {
@@ -1282,7 +1287,7 @@ __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restri
     float vx = NAN, vy = NAN, vz = NAN;
     if (nb.leaf >= 0) {
         float dx, dy, dz, cx, cy, cz;
-        closest_diff(px, py, pz, leaves[nb.leaf], dx, dy, dz, cx, cy, cz);
+        closest_diff(px, py, pz, expand_leaf(leaves[nb.leaf]), dx, dy, dz, cx, cy, cz);
         vx = cx - px;  // closest_point.sub(point)
         vy = cy - py;
         vz = cz - pz;

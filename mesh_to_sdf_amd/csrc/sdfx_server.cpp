@@ -35,7 +35,7 @@ namespace {
 
 // Admission estimates of device bytes (logical sizes; the caching allocator
 // rounds blocks up to powers of two).
-constexpr uint64_t MESH_BYTES_PER_TRI = 96 + 96 + 64;  // topology TriRecs, LBVH leaves, LBVH nodes
+constexpr uint64_t MESH_BYTES_PER_TRI = 96 + 48 + 64;  // topology TriRecs, LBVH leaves, LBVH nodes
 constexpr uint64_t GRID_BYTES_PER_CELL = 28;           // keys, flags, frontier items, ray parity bits
 constexpr uint64_t GRID_BYTES_PER_TRI = 40;            // seed boxes, pair counts, owner table
 constexpr uint64_t GRID_TILE_CELLS = 4096;             // each request's cells start on a tile

@@ -86,10 +86,10 @@ def test_queued_requests_share_batches(sdf):
 
 def test_budget_gates_meshes_and_requests(sdf):
     from mesh_to_sdf_amd import _lib
-    v, f = S.icosphere(3)  # 1,280 triangles: 256 B each resident
+    v, f = S.icosphere(3)  # 1,280 triangles: 208 B each resident
     with _server(budget=2_000_000, workers=2) as srv:
         a = srv.add_mesh(v, sdf.Topology.TriangleList(f))
-        big_v, big_f = S.icosphere(5)  # 20,480 triangles: 5.2 MB resident
+        big_v, big_f = S.icosphere(5)  # 20,480 triangles: 4.3 MB resident
         with pytest.raises(_lib.SdfxError) as e:
             srv.add_mesh(big_v, sdf.Topology.TriangleList(big_f))
         assert e.value.status == _lib.ERR_OVER_BUDGET
