@@ -55,6 +55,7 @@ struct GridDev {
 struct GridReq {
     GridDev g;
     const TriRec* tris;     // the request's triangles, in its topology order
+    LeafRec* ltris;         // the same as 48-byte records (written by k_seed_boxes, read by propagation)
     uint64_t cell_off;      // first cell in the batch's key array (a multiple of COMPACT_TILE)
     uint64_t tri_off;       // first triangle in the batch's triangle numbering
     uint64_t ntris;
@@ -291,6 +292,7 @@ __global__ void __launch_bounds__(256) k_seed_boxes(uint64_t nt, ReqView<mode_ba
     }
     const GridDev g = reqs[q].g;
     const TriRec tr = reqs[q].tris[t - reqs[q].tri_off];
+    reqs[q].ltris[t - reqs[q].tri_off] = make_leafrec(tr);
     Box b;
     tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
     uint32_t mn[3], mx[3];
@@ -604,7 +606,7 @@ __device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
                                           uint32_t q, uint64_t cur, PropAcc& acc) {
     constexpr int SIGN = mode_sign<MODE>;
     const GridReq& R = reqs[q];
-    const TriRec tr = R.tris[tri];
+    const TriRec tr = expand_leaf(R.ltris[tri]);  // three 16-byte loads instead of six, the same values
     const float px = cell_coord(R.g, 0, x), py = cell_coord(R.g, 1, y), pz = cell_coord(R.g, 2, z);
     float dx, dy, dz, cx, cy, cz;
     closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
@@ -1031,6 +1033,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     const uint32_t nreq = (uint32_t)jobs.size();
     // request table: cells padded to whole tiles, rows numbered per axis
     std::vector<GridReq> reqs(nreq);
+    LeafRec* ltris = A.alloc<LeafRec>(nt);
     uint64_t ncells = 0, rows_total[3] = {0, 0, 0}, words_total[3] = {0, 0, 0}, max_extent = 0;
     for (uint32_t q = 0; q < nreq; ++q) {
         const GridBatchItem& J = jobs[q];
@@ -1047,6 +1050,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         R.cell_off = ncells;
         R.ncells = (uint32_t)nc;
         R.tris = J.d_tris;
+        R.ltris = ltris + J.tri_off;
         R.tri_off = J.tri_off;
         R.ntris = J.n_tris;
         R.out = J.d_out;

@@ -37,7 +37,7 @@ namespace {
 // rounds blocks up to powers of two).
 constexpr uint64_t MESH_BYTES_PER_TRI = 96 + 48 + 64;  // topology TriRecs, LBVH leaves, LBVH nodes
 constexpr uint64_t GRID_BYTES_PER_CELL = 28;           // keys, flags, frontier items, ray parity bits
-constexpr uint64_t GRID_BYTES_PER_TRI = 40;            // seed boxes, pair counts, owner table
+constexpr uint64_t GRID_BYTES_PER_TRI = 88;            // seed boxes, pair counts, owner table, 48-byte copies
 constexpr uint64_t GRID_TILE_CELLS = 4096;             // each request's cells start on a tile
 constexpr uint64_t POINT_BYTES_PER_QUERY = 56;         // sort keys, order, radix-sort buffers, deferred searches
 constexpr uint64_t REQUEST_BYTES = 4096;               // tables and flags
