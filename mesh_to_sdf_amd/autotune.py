@@ -14,6 +14,10 @@ Knobs:
   and keeps the largest pair count at which the exhaustive one is still faster.
 * grid_prop_blocks: blocks per grid propagation launch, timed on a 96^3 and a
   128^3 grid.
+* lbvh_visit_budget: node visits after which an LBVH nearest search moves to
+  the per-wave pass, timed with Bvh(Raycast) on the 4M-query noise-sphere
+  workload (327,680 triangles, half the queries near the surface). The best
+  budget grows with the mesh (on an 81,920-triangle sphere 512 beats 1024).
 """
 from __future__ import annotations
 
@@ -84,6 +88,27 @@ def tune_exhaustive(L, sp, log):
     return crossover, rows
 
 
+def tune_visit_budget(L, sp, log, candidates=(384, 512, 768, 1024, 1536, 2048)):
+    import torch
+    v, f, q = S.config3()
+    dv = torch.from_numpy(v).cuda()
+    df = torch.from_numpy(f.reshape(-1).astype(np.int32)).cuda()
+    dq = torch.from_numpy(q).cuda()
+    out = torch.empty(len(q), dtype=torch.float32, device="cuda")
+    n = ctypes.c_uint64(0)
+
+    def call():  # Bvh(Raycast) through the LBVH
+        _lib.check(L.sdfx_generate_sdf_device(dv.data_ptr(), len(v), df.data_ptr(), df.numel(), 0, dq.data_ptr(),
+                                              len(q), 1, 0, out.data_ptr(), ctypes.byref(n), sp))
+    ms = {}
+    with _lib.tuned("exhaustive_max_pairs", 0):
+        for c in candidates:
+            with _lib.tuned("lbvh_visit_budget", c):
+                ms[c] = _timed(call, 5) * 1e3
+            log(f"  budget {c}: {ms[c]:.3f} ms")
+    return min(ms, key=ms.get), {str(c): t for c, t in ms.items()}
+
+
 def tune_grid_blocks(L, sp, log, candidates=(1024, 2048, 4096, 6144, 8192)):
     import torch
     v, f = S.torus()
@@ -117,12 +142,16 @@ def tune(out_path: str = DEFAULT_PATH, log=print) -> dict:
     sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     log("exhaustive_max_pairs:")
     crossover, rows = tune_exhaustive(L, sp, log)
+    log("lbvh_visit_budget:")
+    budget, budget_ms = tune_visit_budget(L, sp, log)
     log("grid_prop_blocks:")
     blocks, grid_ms = tune_grid_blocks(L, sp, log)
     result = {
         "device": torch.cuda.get_device_name(0),
-        "knobs": {"exhaustive_max_pairs": int(crossover), "grid_prop_blocks": int(blocks)},
-        "measurements": {"exhaustive_vs_lbvh": rows, "grid_ms_by_blocks": grid_ms},
+        "knobs": {"exhaustive_max_pairs": int(crossover), "grid_prop_blocks": int(blocks),
+                  "lbvh_visit_budget": int(budget)},
+        "measurements": {"exhaustive_vs_lbvh": rows, "lbvh_ms_by_visit_budget": budget_ms,
+                         "grid_ms_by_blocks": grid_ms},
     }
     if out_path:
         with open(out_path, "w") as fh:
