@@ -199,7 +199,8 @@ sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex
  *   down to a multiple of 8; default 4096).
  *   "lbvh_visit_budget": LBVH node visits after which a nearest search stops
  *   in the per-query pass and finishes in a second pass that gives the query a
- *   whole wave, 16..2^31-1 (default 1024). */
+ *   whole wave, 16..2^31-1, or 0 (the default) for 15 x cbrt(triangles) per
+ *   mesh. */
 int sdfx_set_tuning(const char* key, int64_t value);
 int64_t sdfx_get_tuning(const char* key);
 

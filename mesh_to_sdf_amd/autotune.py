@@ -15,9 +15,10 @@ Knobs:
 * grid_prop_blocks: blocks per grid propagation launch, timed on a 96^3 and a
   128^3 grid.
 * lbvh_visit_budget: node visits after which an LBVH nearest search moves to
-  the per-wave pass, timed with Bvh(Raycast) on the 4M-query noise-sphere
-  workload (327,680 triangles, half the queries near the surface). The best
-  budget grows with the mesh (on an 81,920-triangle sphere 512 beats 1024).
+  the per-wave pass. 0 (the library's default) picks 15 x cbrt(triangles) per
+  mesh; the tuner times it against fixed budgets with Bvh(Raycast) on the
+  4M-query noise-sphere workload (327,680 triangles, half the queries near the
+  surface) and keeps 0 unless a fixed budget is more than 3% faster.
 """
 from __future__ import annotations
 
@@ -88,7 +89,7 @@ def tune_exhaustive(L, sp, log):
     return crossover, rows
 
 
-def tune_visit_budget(L, sp, log, candidates=(384, 512, 768, 1024, 1536, 2048)):
+def tune_visit_budget(L, sp, log, candidates=(0, 384, 512, 768, 1024, 1536, 2048)):
     import torch
     v, f, q = S.config3()
     dv = torch.from_numpy(v).cuda()
@@ -105,8 +106,9 @@ def tune_visit_budget(L, sp, log, candidates=(384, 512, 768, 1024, 1536, 2048)):
         for c in candidates:
             with _lib.tuned("lbvh_visit_budget", c):
                 ms[c] = _timed(call, 5) * 1e3
-            log(f"  budget {c}: {ms[c]:.3f} ms")
-    return min(ms, key=ms.get), {str(c): t for c, t in ms.items()}
+            log(f"  budget {c or 'by mesh size'}: {ms[c]:.3f} ms")
+    best = min(ms, key=ms.get)
+    return (0 if ms[0] <= 1.03 * ms[best] else best), {str(c): t for c, t in ms.items()}
 
 
 def tune_grid_blocks(L, sp, log, candidates=(1024, 2048, 4096, 6144, 8192)):

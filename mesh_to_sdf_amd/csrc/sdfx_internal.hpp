@@ -143,7 +143,7 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits, Aren
 struct Tuning {
     int64_t exhaustive_max_pairs;
     int64_t grid_prop_blocks;
-    int64_t lbvh_visit_budget;  // node visits after which a nearest search moves to the per-wave pass
+    int64_t lbvh_visit_budget;  // node visits after which a nearest search moves to the per-wave pass (0: by mesh size)
 };
 Tuning& tuning();
 

@@ -869,7 +869,7 @@ struct PointReq {
     int32_t root;
     uint32_t max_visits;
     uint32_t flag;
-    uint32_t pad;
+    uint32_t budget;  // nearest-search visits before the per-wave pass (batched queries)
 };
 
 // Entries per wave in the work pool of k_lbvh_coop (node, box d^2 bits).
@@ -1008,6 +1008,16 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
 
 static uint32_t visit_cap(const Lbvh& bvh) { return (uint32_t)(2 * bvh.n + 64); }
 
+// Node visits after which a nearest search moves to the per-wave pass: the
+// tuning knob, or with 0 (the default) 15 x cbrt(triangles), which measured
+// best with Bvh(Raycast) on noise spheres of 5,120 to 327,680 triangles
+// (budgets 256 to 1024; half near-surface, half uniform queries).
+static uint32_t visit_budget(const Lbvh& bvh) {
+    const int64_t k = tuning().lbvh_visit_budget;
+    if (k > 0) return (uint32_t)k;
+    return (uint32_t)std::max(64.0, 15.0 * std::cbrt((double)bvh.n));
+}
+
 // Morton code of each query inside the tree's root box (clamped outside it).
 __global__ void __launch_bounds__(256) k_query_morton(const BvhNode* __restrict__ nodes, const float* __restrict__ q,
                                                       uint64_t nq, uint32_t* __restrict__ keys,
@@ -1081,7 +1091,7 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     // that gives each of them a whole wave: a few interior queries can need
     // tens of thousands of visits, and one lane walking them alone would set
     // the pass's duration.
-    const uint64_t budget = (uint64_t)std::max<int64_t>(tuning().lbvh_visit_budget, 1);
+    const uint64_t budget = visit_budget(bvh);
     const bool split = mode != QM_NORMAL_FOLD && bvh.root >= 0 && budget < visit_cap(bvh);
     DeferRec* recs = nullptr;
     uint32_t* words = nullptr;  // n_deferred, next
@@ -1179,8 +1189,8 @@ __global__ void __launch_bounds__(256) k_point_morton_batch(const PointReq* __re
 template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
                                                           uint64_t total, const uint32_t* __restrict__ order,
-                                                          DevFlags* flags, uint32_t budget,
-                                                          DeferRec* __restrict__ deferred, uint32_t* n_deferred) {
+                                                          DevFlags* flags, DeferRec* __restrict__ deferred,
+                                                          uint32_t* n_deferred) {
     const uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
     bool dfr = false;
     DeferRec rec;
@@ -1190,7 +1200,7 @@ __global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __rest
         const PointReq& R = reqs[r];
         TravStats ts;
         dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag, ts,
-                                budget, deferred ? &rec : nullptr);
+                                R.budget, deferred ? &rec : nullptr);
         rec.req = r;
     }
     if (deferred) {  // kernel argument: uniform
@@ -1217,7 +1227,7 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
             const PointBatchItem& it = items[k];
             if (!it.nq || !it.bvh || !it.bvh->n) continue;
             h.push_back(PointReq{it.bvh->nodes.get(), it.bvh->leaves.get(), it.d_q, it.d_out, total, it.bvh->root,
-                                 visit_cap(*it.bvh), it.flag, 0u});
+                                 visit_cap(*it.bvh), it.flag, visit_budget(*it.bvh)});
             total += it.nq;
         }
         if (h.empty()) continue;
@@ -1239,7 +1249,6 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
         const unsigned blocks = order ? xcd_grid(blocks_for(total, 256)) : blocks_for(total, 256);
         // long nearest searches finish one per wave, as in lbvh_query
         const bool split = mode != QM_NORMAL_FOLD;
-        const uint32_t budget = (uint32_t)std::max<int64_t>(tuning().lbvh_visit_budget, 1);
         DeferRec* recs = nullptr;
         uint32_t* words = nullptr;  // n_deferred, next
         if (split) {
@@ -1250,19 +1259,19 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
         switch (mode) {
             case QM_RAYCAST_VOTE:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags, budget, recs, words);
+                            order, d_flags, recs, words);
                 break;
             case QM_NORMAL_FOLD:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags, budget, recs, words);
+                            order, d_flags, recs, words);
                 break;
             case QM_NEAREST_SIGNED:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
-                            total, order, d_flags, budget, recs, words);
+                            total, order, d_flags, recs, words);
                 break;
             case QM_NEAREST_VOTE:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags, budget, recs, words);
+                            order, d_flags, recs, words);
                 break;
         }
         if (split)

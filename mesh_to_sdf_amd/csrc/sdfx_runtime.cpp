@@ -156,7 +156,7 @@ namespace sdfx {
 Tuning& tuning() {
     // 2^26 pairs: about 10k queries against 6k triangles, where the exhaustive
     // kernel finishes well before an LBVH build and traversal would.
-    static Tuning t{int64_t(1) << 26, 4096, 1024};
+    static Tuning t{int64_t(1) << 26, 4096, 0};
     return t;
 }
 
@@ -282,7 +282,7 @@ int sdfx_set_tuning(const char* key, int64_t value) {
     int64_t* slot = tuning_slot(key);
     if (!slot) return -1;
     if (std::string(key) == "grid_prop_blocks" && (value < 8 || value > 65536)) return -1;
-    if (std::string(key) == "lbvh_visit_budget" && (value < 16 || value > INT32_MAX)) return -1;
+    if (std::string(key) == "lbvh_visit_budget" && value != 0 && (value < 16 || value > INT32_MAX)) return -1;
     *slot = value;
     return 0;
 }

@@ -75,4 +75,7 @@ def test_tuning_knobs_and_ranges():
             assert L.sdfx_set_tuning(key.encode(), bad) == -1 and L.sdfx_get_tuning(key.encode()) == good
         assert L.sdfx_set_tuning(key.encode(), old) == 0
     assert L.sdfx_set_tuning(b"lbvh_visit_budget", 1 << 31) == -1
+    old = L.sdfx_get_tuning(b"lbvh_visit_budget")
+    assert L.sdfx_set_tuning(b"lbvh_visit_budget", 0) == 0  # 0: chosen per mesh from its size
+    assert L.sdfx_set_tuning(b"lbvh_visit_budget", old) == 0
     assert L.sdfx_set_tuning(b"no_such_knob", 1) == -1 and L.sdfx_get_tuning(b"no_such_knob") == -1
