@@ -1233,7 +1233,7 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
         if (h.empty()) continue;
         if (total >= (1ull << 32)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "a point batch holds fewer than 2^32 queries");
         PointReq* d_reqs = A.alloc<PointReq>(h.size());
-        SDFX_HIP(hipMemcpyAsync(d_reqs, h.data(), h.size() * sizeof(PointReq), hipMemcpyHostToDevice, s));
+        A.upload(d_reqs, h.data(), h.size() * sizeof(PointReq));
         const uint32_t nreq = (uint32_t)h.size();
         const uint32_t* order = nullptr;
         if (total >= 4096) {

@@ -147,6 +147,56 @@ void dev_free(void* p) {
     P.live.erase(it);
 }
 
+namespace {
+struct PinnedPool {
+    std::mutex mu;
+    std::map<size_t, std::vector<void*>> free_blocks;  // size class -> blocks
+    std::unordered_map<void*, size_t> live;
+};
+PinnedPool& pinned_pool() {
+    static PinnedPool* p = new PinnedPool();  // intentionally leaked: blocks live until process exit
+    return *p;
+}
+}  // namespace
+
+void* pinned_alloc(size_t bytes) {
+    const size_t cls = size_class(bytes ? bytes : 1);
+    PinnedPool& P = pinned_pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto& fl = P.free_blocks[cls];
+        if (!fl.empty()) {
+            void* p = fl.back();
+            fl.pop_back();
+            P.live[p] = cls;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    SDFX_HIP(hipHostMalloc(&p, cls, hipHostMallocDefault));
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.live[p] = cls;
+    return p;
+}
+
+void pinned_free(void* p) {
+    if (!p) return;
+    PinnedPool& P = pinned_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.live.find(p);
+    if (it == P.live.end()) return;
+    P.free_blocks[it->second].push_back(p);
+    P.live.erase(it);
+}
+
+void Arena::upload(void* d_dst, const void* h_src, size_t bytes) {
+    if (!bytes) return;
+    void* h = pinned_alloc(bytes);
+    staged.push_back(h);
+    std::memcpy(h, h_src, bytes);
+    SDFX_HIP(hipMemcpyAsync(d_dst, h, bytes, hipMemcpyHostToDevice, stream));
+}
+
 // ---------------------------------------------------------------- tuning
 
 }  // namespace sdfx

@@ -42,6 +42,10 @@ const std::string& last_error();
 // lazily with the default priority.
 hipStream_t thread_stream();
 
+// Pinned host blocks from a process-wide cache (hipHostMalloc, kept until exit).
+void* pinned_alloc(size_t bytes);
+void pinned_free(void* p);
+
 // Caching device allocator. Blocks are returned to a per-device free list and
 // reused by later calls; memory is released only at process exit.
 void* dev_alloc(size_t bytes);
@@ -92,6 +96,7 @@ struct DevBuf {
 struct Arena {
     hipStream_t stream;
     std::vector<void*> blocks;
+    std::vector<void*> staged;  // pinned host blocks of upload()
     explicit Arena(hipStream_t s) : stream(s) {}
     Arena(const Arena&) = delete;
     Arena& operator=(const Arena&) = delete;
@@ -101,9 +106,14 @@ struct Arena {
         blocks.push_back(p);
         return static_cast<T*>(p);
     }
+    // Copies `bytes` of host memory to d_dst on the stream without blocking the
+    // host: the data goes through a pinned block the Arena keeps until it has
+    // synchronised.
+    void upload(void* d_dst, const void* h_src, size_t bytes);
     ~Arena() {
         (void)hipStreamSynchronize(stream);
         for (void* p : blocks) dev_free(p);
+        for (void* p : staged) pinned_free(p);
     }
 };
 

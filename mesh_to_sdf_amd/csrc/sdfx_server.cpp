@@ -134,16 +134,29 @@ struct sdfx_server {
     }
 
     void worker_main();
-    void run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, hipStream_t s);
+    // Per worker: the grid stream, the point stream and two events ordering them.
+    struct WorkerStreams {
+        hipStream_t grid = nullptr, points = nullptr;
+        hipEvent_t flags_ready = nullptr, points_done = nullptr;
+    };
+    void run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, const WorkerStreams& ws);
 };
 
-void sdfx_server::run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, hipStream_t s) {
+// A batch's point requests (None and LBVH) run on the worker's point stream
+// while the host drives the grid rounds on its grid stream: the grid rounds are
+// bound by launch and memory latency and leave most of the GPU to the point
+// traversals. The two streams meet again before the flags are read.
+void sdfx_server::run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, const WorkerStreams& ws) {
     const size_t n = batch.size();
+    const hipStream_t s = ws.grid;
     std::vector<DevFlags> hf(n);
     {
         Arena A(s);
+        Arena B(ws.points);  // destroyed first: synchronises the point stream before A's memory is released
         DevFlags* flags = A.alloc<DevFlags>(n);
         fill_u32(reinterpret_cast<uint32_t*>(flags), n * (sizeof(DevFlags) / sizeof(uint32_t)), 0u, s);
+        SDFX_HIP(hipEventRecord(ws.flags_ready, s));
+        SDFX_HIP(hipStreamWaitEvent(ws.points, ws.flags_ready, 0));
         std::vector<GridBatchItem> grid[2];
         uint64_t tri_off[2] = {0, 0};
         std::vector<PointBatchItem> points[4];
@@ -156,16 +169,18 @@ void sdfx_server::run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, 
                 grid[g].push_back(GridBatchItem{m.tris.get(), tri_off[g], m.nt, r.gp, r.d_out, (uint32_t)i});
                 tri_off[g] += m.nt;
             } else if (r.method == SDFX_ACCEL_NONE) {
-                none_query(m.tris.get(), m.nt, r.d_q, r.nq, r.sign, r.d_out, flags + i, A);
+                none_query(m.tris.get(), m.nt, r.d_q, r.nq, r.sign, r.d_out, flags + i, B);
             } else if (m.nt > 0 && r.nq > 0) {
                 points[point_mode(r.method, r.sign)].push_back(PointBatchItem{&m.bvh, r.d_q, r.nq, r.d_out,
                                                                               (uint32_t)i});
             }
         }
+        for (int q = 0; q < 4; ++q)  // queued on the point stream without blocking the host
+            if (!points[q].empty()) lbvh_query_batch(points[q], (QueryMode)q, flags, B);
+        SDFX_HIP(hipEventRecord(ws.points_done, ws.points));
         for (int g = 0; g < 2; ++g)
             if (!grid[g].empty()) grid_sdf_batch(grid[g], g ? SDFX_SIGN_NORMAL : SDFX_SIGN_RAYCAST, flags, A);
-        for (int q = 0; q < 4; ++q)
-            if (!points[q].empty()) lbvh_query_batch(points[q], (QueryMode)q, flags, A);
+        SDFX_HIP(hipStreamWaitEvent(s, ws.points_done, 0));
         SDFX_HIP(hipMemcpyAsync(hf.data(), flags, n * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
         SDFX_HIP(hipStreamSynchronize(s));
     }
@@ -186,12 +201,17 @@ void sdfx_server::run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, 
 }
 
 void sdfx_server::worker_main() {
-    hipStream_t s = nullptr;
+    WorkerStreams ws;
+    bool ready = false;
     try {
         SDFX_HIP(hipSetDevice(device));
-        SDFX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        SDFX_HIP(hipStreamCreateWithFlags(&ws.grid, hipStreamNonBlocking));
+        SDFX_HIP(hipStreamCreateWithFlags(&ws.points, hipStreamNonBlocking));
+        SDFX_HIP(hipEventCreateWithFlags(&ws.flags_ready, hipEventDisableTiming));
+        SDFX_HIP(hipEventCreateWithFlags(&ws.points_done, hipEventDisableTiming));
+        ready = true;
     } catch (const Error&) {
-        s = nullptr;
+        ready = false;
     }
     for (;;) {
         std::unique_lock<std::mutex> lk(mu);
@@ -202,12 +222,12 @@ void sdfx_server::worker_main() {
         lk.unlock();
         sdfx_status st = SDFX_OK;
         std::string msg;
-        if (!s) {
+        if (!ready) {
             st = SDFX_ERR_DEVICE;
-            msg = "server worker could not create its stream";
+            msg = "server worker could not create its streams";
         } else {
             try {
-                run_batch(batch, s);
+                run_batch(batch, ws);
             } catch (const Error& e) {
                 st = e.status;
                 msg = e.what();
@@ -235,7 +255,10 @@ void sdfx_server::worker_main() {
         cv_done.notify_all();
         cv_work.notify_all();
     }
-    if (s) (void)hipStreamDestroy(s);
+    if (ws.flags_ready) (void)hipEventDestroy(ws.flags_ready);
+    if (ws.points_done) (void)hipEventDestroy(ws.points_done);
+    if (ws.points) (void)hipStreamDestroy(ws.points);
+    if (ws.grid) (void)hipStreamDestroy(ws.grid);
 }
 
 namespace {
