@@ -1,0 +1,45 @@
+"""The benchmark workloads at their full sizes, checked through properties that
+do not need the oracle to run at that size:
+
+* the noise-sphere point workload (327,680 triangles, 4,194,304 queries): the
+  LBVH path (per-query pass, then the per-wave pass for searches past the visit
+  budget) gives the same bits as the exhaustive kernel, an independent
+  algorithm that evaluates every triangle for every query, and every answer is
+  finite;
+* the batched grid workload (256 meshes, 84.3M cells): every request of one
+  batched call gets the same bits as the same request alone.
+The oracle pins both paths at smaller sizes (test_gpu_parity.py,
+test_gpu_grid_batch.py)."""
+import numpy as np
+import pytest
+
+from mesh_to_sdf_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("method", ["Bvh", "Rtree"])
+def test_noise_sphere_4m_lbvh_equals_exhaustive(sdf, method):
+    v, f, q = S.config3()
+    T = sdf.Topology.TriangleList(f)
+    m = sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast) if method == "Bvh" else sdf.AccelerationMethod.Rtree
+    with sdf._lib.tuned("exhaustive_max_pairs", 0):
+        lbvh = sdf.generate_sdf(v, T, q, m)
+    with sdf._lib.tuned("exhaustive_max_pairs", 1 << 62):
+        exhaustive = sdf.generate_sdf(v, T, q, m)
+    assert np.all(np.isfinite(lbvh))
+    bad = np.flatnonzero(bits(lbvh) != bits(exhaustive))
+    assert bad.size == 0, f"{bad.size} of {len(q)} differ, first {bad[:5]}"
+
+
+def test_batched_grids_256_equal_alone(sdf):
+    reqs = [(v, sdf.Topology.TriangleList(f), sdf.Grid(first, size, count))
+            for v, f, first, size, count in S.config4()]
+    batch = sdf.generate_grid_sdf_batch(reqs, sdf.SignMethod.Raycast)
+    for i, (v, topo, grid) in enumerate(reqs):
+        alone = sdf.generate_grid_sdf(v, topo, grid, sdf.SignMethod.Raycast)
+        assert np.array_equal(bits(batch[i]), bits(alone)), f"request {i}"
