@@ -527,31 +527,23 @@ __device__ __forceinline__ void visit_leaf_nearest(const LeafRec* __restrict__ l
 }
 
 // Nearest search of one lane from `start` (a node, or ~leaf), closest child
-// first, folding into the lane's running best. With SHARED, nodes are pruned
-// against min(best, *shared), a word in LDS that the lanes of a wave lower
-// together (atomicMin of the d^2 bits: d^2 >= 0, so bit order is float order).
-// Returns false when more than `budget` nodes were visited with nodes left
-// (the search is incomplete; best holds what was found so far) if budget is
-// below `cap`; more than `cap` visits means a malformed tree (overflow).
+// first, folding into the lane's running best. Returns false when more than
+// `budget` nodes were visited with nodes left (the search is incomplete; best
+// holds what was found so far) if budget is below `cap`; more than `cap`
+// visits means a malformed tree (overflow).
 //
 // Pruning keeps a box when box_d2 * PRUNE_SLACK <= bound, and bound is always
 // the d^2 of a triangle already seen, so a box is dropped only if every
 // triangle in it is strictly farther than one already seen. The answer is
 // therefore the minimum of (d^2, topology index) over all triangles whatever
-// the visiting order, start nodes or initial bound: the searches below that
-// split one query's tree among lanes return the same answer as one lane.
-template <bool SHARED>
+// the visiting order, start nodes or initial bound: k_lbvh_coop, which splits
+// one query's tree among the lanes of a wave, returns the same answer as one
+// lane.
 __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t start,
                              float px, float py, float pz, uint32_t budget, uint32_t cap, Nearest& best,
-                             uint32_t* shared, bool& overflow, TravStats& ts) {
-    auto bound = [&]() {
-        return SHARED ? fminf(best.d2, bitsf(*(volatile uint32_t*)shared)) : best.d2;
-    };
-    auto leaf = [&](int32_t lf) {
-        const float before = best.d2;
-        visit_leaf_nearest(leaves, lf, px, py, pz, best, ts);
-        if (SHARED && best.d2 < before) atomicMin(shared, fbits(best.d2));
-    };
+                             bool& overflow, TravStats& ts) {
+    auto bound = [&]() { return best.d2; };
+    auto leaf = [&](int32_t lf) { visit_leaf_nearest(leaves, lf, px, py, pz, best, ts); };
     if (start < 0) {
         leaf(~start);
         return true;
@@ -616,7 +608,7 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* _
 __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                                float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
     Nearest best{INFINITY, 0xFFFFFFFFu, -1};
-    nearest_from<false>(nodes, leaves, root, px, py, pz, max_visits, max_visits, best, nullptr, overflow, ts);
+    nearest_from(nodes, leaves, root, px, py, pz, max_visits, max_visits, best, overflow, ts);
     return best;
 }
 
@@ -797,8 +789,8 @@ __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, c
         if (nan_seen) atomicOr(&flags->nan_distance, 1u);
     } else {
         Nearest nb{INFINITY, 0xFFFFFFFFu, -1};
-        const bool done = nearest_from<false>(nodes, leaves, root, px, py, pz, defer ? min(budget, max_visits) : max_visits,
-                                              max_visits, nb, nullptr, overflow, ts);
+        const bool done = nearest_from(nodes, leaves, root, px, py, pz, defer ? min(budget, max_visits) : max_visits,
+                                       max_visits, nb, overflow, ts);
         const bool inside = MODE != QM_NEAREST_SIGNED &&
                             bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
         if (done) {
@@ -881,7 +873,7 @@ constexpr uint32_t COOP_POOL = 128;
 // lanes donate to a pool in LDS while the pool holds fewer than 64 entries,
 // and nodes are pruned against the wave's minimum d^2. The answer is the
 // wave's minimum of (d^2, topology index), starting from the best triangle
-// the first pass found (see nearest_from for why the order does not matter).
+// the first pass found (nearest_from explains why the order does not matter).
 // Persistent waves take queries from *next until all *n_deferred are done.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ reqs, PointReq single,
