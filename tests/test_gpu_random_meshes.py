@@ -58,3 +58,39 @@ def test_random_soups_all_methods(sdf, oracle, seed, path):
             gpu = sdf.generate_sdf(v, T, q, _accel(sdf, accel, sign))
             bad = np.flatnonzero(bits(gpu) != bits(ref))
             assert bad.size == 0, (seed, path, accel, sign, bad[:5], gpu[bad[:5]], ref[bad[:5]])
+
+
+def random_grid(g, v):
+    lo, hi = v.min(0), v.max(0)
+    count = [int(c) for c in g.integers(1, 24, size=3)]
+    scale = np.float32(g.uniform(1.0, 1.4))
+    size = ((hi - lo) * scale / np.array(count, np.float32)).astype(np.float32)
+    first = (lo - size * np.float32(0.5)).astype(np.float32)
+    flip = g.integers(0, 4)
+    if flip < 3:  # a negative cell size on one axis: the grid runs backwards from its first cell
+        first[flip] = hi[flip] + size[flip] * np.float32(0.5)
+        size[flip] = -size[flip]
+    return first, size, count
+
+
+@pytest.mark.parametrize("sign", [0, 1])
+@pytest.mark.parametrize("seed", [4, 5, 6])
+def test_random_soups_grids(sdf, oracle, seed, sign):
+    """Degenerate soups on random small grids (one-cell axes, negative cell
+    sizes), alone and batched, against the oracle's restatement of the device
+    schedule."""
+    g = np.random.default_rng(seed)
+    reqs, refs = [], []
+    for k in range(4):
+        v, f, _ = soup(seed * 10 + k, n_tris=120)
+        first, size, count = random_grid(g, v)
+        s, ref, _ = oracle.generate_grid_sdf(v, f.reshape(-1), oracle.TOPO_LIST, first, size, count, sign, 8,
+                                             oracle.GRID_FRONTIER)
+        assert s == 0
+        reqs.append((v, sdf.Topology.TriangleList(f.reshape(-1)), sdf.Grid(first, size, count)))
+        refs.append(ref)
+    batch = sdf.generate_grid_sdf_batch(reqs, sign)
+    for k, ((v, topo, grid), ref) in enumerate(zip(reqs, refs)):
+        alone = sdf.generate_grid_sdf(v, topo, grid, sign)
+        assert np.array_equal(bits(alone), bits(ref)), (seed, sign, k, "alone")
+        assert np.array_equal(bits(batch[k]), bits(ref)), (seed, sign, k, "batched")
