@@ -88,6 +88,7 @@ enum QueryMode : int {
     QM_NORMAL_FOLD = 1,      // compare_distances fold: Bvh(Normal)
     QM_NEAREST_SIGNED = 2,   // nearest by d^2, signed distance to it: Rtree
     QM_NEAREST_VOTE = 3,     // nearest by d^2, distance to it + 3-ray vote: RtreeBvh
+    QM_CLOSEST_VECTOR = 4,   // nearest by d^2, vector to its closest point (query_vector_to_closest_point)
 };
 
 void lbvh_query(const Lbvh& bvh, const float* d_queries, uint64_t n_queries, QueryMode mode, float* d_out,

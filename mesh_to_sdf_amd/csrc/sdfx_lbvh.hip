@@ -748,6 +748,23 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const LeafRec
     return insides > 1;
 }
 
+// generic/rtree_bvh.rs:217-276: closest_point.sub(point) of the nearest
+// triangle (NaN without one).
+__device__ __forceinline__ void closest_vector(const Nearest& nb, float px, float py, float pz,
+                                               const LeafRec* __restrict__ leaves, float* __restrict__ v3) {
+    float vx = NAN, vy = NAN, vz = NAN;
+    if (nb.leaf >= 0) {
+        float dx, dy, dz, cx, cy, cz;
+        closest_diff(px, py, pz, expand_leaf(leaves[nb.leaf]), dx, dy, dz, cx, cy, cz);
+        vx = cx - px;
+        vy = cy - py;
+        vz = cz - pz;
+    }
+    v3[0] = vx;
+    v3[1] = vy;
+    v3[2] = vz;
+}
+
 // The answer of a nearest-search mode from the nearest triangle (d^2, leaf)
 // and the 3-ray vote.
 template <int MODE>
@@ -988,7 +1005,10 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
         }
         const int wl = __ffsll((unsigned long long)__ballot(key == mk)) - 1;
         Nearest nb{bitsf((uint32_t)(mk >> 32)), (uint32_t)mk, __shfl(best.leaf, wl, 64)};
-        if (lane == 0) R.out[r.q] = nearest_answer<MODE>(nb, r.inside != 0u, px, py, pz, leaves);
+        if (lane == 0) {
+            if constexpr (MODE == QM_CLOSEST_VECTOR) closest_vector(nb, px, py, pz, leaves, R.out + 3 * r.q);
+            else R.out[r.q] = nearest_answer<MODE>(nb, r.inside != 0u, px, py, pz, leaves);
+        }
         if (__ballot(overflow) && lane == 0) atomicOr(&flags[R.flag].internal, 1u);
         overflow = false;
     }
@@ -1066,6 +1086,10 @@ static void launch_coop(QueryMode mode, const PointReq* reqs, const PointReq& si
             break;
         case QM_NEAREST_VOTE:
             SDFX_LAUNCH(k_lbvh_coop<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, reqs, single, recs, words,
+                        words + 1, d_flags, stats);
+            break;
+        case QM_CLOSEST_VECTOR:
+            SDFX_LAUNCH(k_lbvh_coop<QM_CLOSEST_VECTOR>, dim3(blocks), dim3(256), 0, s, reqs, single, recs, words,
                         words + 1, d_flags, stats);
             break;
         default:
@@ -1273,31 +1297,44 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
     }
 }
 
+// query_vector_to_closest_point: the vector to the nearest triangle's closest
+// point and the 3-ray vote per query; nearest searches past `budget` visits
+// are deferred to k_lbvh_coop (which writes their vectors) as in lbvh_query.
 __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restrict__ nodes,
                                                          const LeafRec* __restrict__ leaves, int32_t root,
                                                          uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                          const uint32_t* __restrict__ order, float* __restrict__ vec,
-                                                         uint8_t* __restrict__ inside, DevFlags* flags) {
+                                                         uint8_t* __restrict__ inside, DevFlags* flags,
+                                                         uint32_t budget, DeferRec* __restrict__ deferred,
+                                                         uint32_t* n_deferred) {
     uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    if (order) i = order[i];
-    float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
-    bool overflow = false;
-    TravStats ts;
-    Nearest nb = bvh_nearest(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
-    float vx = NAN, vy = NAN, vz = NAN;
-    if (nb.leaf >= 0) {
-        float dx, dy, dz, cx, cy, cz;
-        closest_diff(px, py, pz, expand_leaf(leaves[nb.leaf]), dx, dy, dz, cx, cy, cz);
-        vx = cx - px;  // closest_point.sub(point)
-        vy = cy - py;
-        vz = cz - pz;
+    bool dfr = false;
+    DeferRec rec;
+    if (i < nq) {  // no early return: the wave appends its deferred queries together
+        if (order) i = order[i];
+        const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+        bool overflow = false;
+        TravStats ts;
+        Nearest nb{INFINITY, 0xFFFFFFFFu, -1};
+        const bool done = nearest_from(nodes, leaves, root, px, py, pz, deferred ? min(budget, max_visits) : max_visits,
+                                       max_visits, nb, overflow, ts);
+        if (done) closest_vector(nb, px, py, pz, leaves, vec + 3 * i);
+        else rec = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, 0u, 0u, 0u};
+        dfr = !done;
+        inside[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? 1 : 0;
+        if (overflow) atomicOr(&flags->internal, 1u);
     }
-    vec[3 * i] = vx;
-    vec[3 * i + 1] = vy;
-    vec[3 * i + 2] = vz;
-    inside[i] = bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts) ? 1 : 0;
-    if (overflow) atomicOr(&flags->internal, 1u);
+    if (deferred) {  // kernel argument: uniform; one atomic per wave appends its deferred queries
+        const uint64_t m = __ballot(dfr);
+        if (m) {
+            const uint32_t lane = threadIdx.x & 63u;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(n_deferred, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (dfr) deferred[base + __popcll(m & ((1ull << lane) - 1ull))] = rec;
+        }
+    }
 }
 
 void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float* d_vec, uint8_t* d_inside,
@@ -1306,8 +1343,22 @@ void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float*
     hipStream_t s = A.stream;
     const uint32_t* order = query_order(bvh, d_q, nq, A);
     const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
-    SDFX_LAUNCH(k_closest_vectors, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(),
-                       bvh.leaves.get(), bvh.root, visit_cap(bvh), d_q, nq, order, d_vec, d_inside, d_flags);
+    const uint32_t budget = visit_budget(bvh);
+    const bool split = bvh.root >= 0 && budget < visit_cap(bvh);
+    DeferRec* recs = nullptr;
+    uint32_t* words = nullptr;  // n_deferred, next
+    if (split) {
+        recs = A.alloc<DeferRec>(nq);
+        words = A.alloc<uint32_t>(2);
+        fill_u32(words, 2, 0u, s);
+    }
+    SDFX_LAUNCH(k_closest_vectors, dim3(blocks), dim3(256), 0, s, bvh.nodes.get(), bvh.leaves.get(), bvh.root,
+                visit_cap(bvh), d_q, nq, order, d_vec, d_inside, d_flags, budget, recs, words);
+    if (split) {
+        const PointReq single{bvh.nodes.get(), bvh.leaves.get(), d_q, d_vec, 0, bvh.root, visit_cap(bvh), 0u, 0u};
+        launch_coop(QM_CLOSEST_VECTOR, nullptr, single, words, recs, (unsigned)std::min<uint64_t>(2048, (nq + 3) / 4),
+                    d_flags, nullptr, s);
+    }
     SDFX_HIP(hipGetLastError());
 }
 

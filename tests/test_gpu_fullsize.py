@@ -43,3 +43,21 @@ def test_batched_grids_256_equal_alone(sdf):
     for i, (v, topo, grid) in enumerate(reqs):
         alone = sdf.generate_grid_sdf(v, topo, grid, sdf.SignMethod.Raycast)
         assert np.array_equal(bits(batch[i]), bits(alone)), f"request {i}"
+
+
+def test_noise_sphere_4m_closest_vectors_finite_and_consistent(sdf):
+    """query_vector_to_closest_point on the 4M-query workload: every vector is
+    finite and its length is the RtreeBvh distance's magnitude (both come from
+    the same nearest triangle), with and without the per-wave pass."""
+    v, f, q = S.config3()
+    T = sdf.Topology.TriangleList(f)
+    acc = sdf.build_sdf_acceleration(v, T, sdf.AccelerationMethod.RtreeBvh)
+    vec, inside = sdf.query_vector_to_closest_point_rtree_bvh(acc, q)
+    with sdf._lib.tuned("lbvh_visit_budget", (1 << 31) - 1):
+        vec1, inside1 = sdf.query_vector_to_closest_point_rtree_bvh(acc, q)
+    assert np.array_equal(bits(vec), bits(vec1)) and np.array_equal(inside, inside1)
+    assert np.all(np.isfinite(vec))
+    d = sdf.query_sdf(acc, q)
+    assert np.array_equal(inside.astype(bool), d < 0)
+    assert np.allclose(np.linalg.norm(vec.astype(np.float64), axis=1), np.abs(d), rtol=1e-5, atol=1e-7)
+    acc.close()

@@ -137,12 +137,16 @@ def test_resident_mesh_matches_generate(sdf, oracle, path):
         acc.close()
 
 
-def test_closest_vectors_bit_exact(sdf, oracle):
+@pytest.mark.parametrize("budget", [0, 16])
+def test_closest_vectors_bit_exact(sdf, oracle, budget):
+    """With the per-mesh visit budget and with a 16-visit one (nearly every
+    search finished by the per-wave pass)."""
     v, f = S.torus(1.0, 0.35, 32, 16, 0.002)
     q = S.uniform_queries(v, 4000, 106)
     s, vec_ref, in_ref = oracle.query_vector_to_closest_point(v, f, oracle.TOPO_LIST, q)
     acc = sdf.build_sdf_acceleration(v, sdf.Topology.TriangleList(f), sdf.AccelerationMethod.RtreeBvh)
-    vec, inside = sdf.query_vector_to_closest_point_rtree_bvh(acc, q)
+    with sdf._lib.tuned("lbvh_visit_budget", budget):
+        vec, inside = sdf.query_vector_to_closest_point_rtree_bvh(acc, q)
     assert np.array_equal(vec.view(np.uint32), vec_ref.view(np.uint32))
     assert np.array_equal(inside, in_ref)
 
