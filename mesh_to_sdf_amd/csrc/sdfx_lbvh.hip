@@ -1136,6 +1136,8 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         case QM_NEAREST_VOTE:
             launch_query<QM_NEAREST_VOTE>(bvh, d_q, nq, order, blocks, d_out, d_flags, stats, b32, recs, words, s);
             break;
+        case QM_CLOSEST_VECTOR:  // vectors have their own entry point, lbvh_closest_vectors
+            throw Error(SDFX_ERR_INVALID_ARGUMENT, "closest-point vectors are not an SDF query mode");
     }
     unsigned long long* cstats = stats ? stats + 6ull * blocks : nullptr;
     if (split) {
@@ -1289,6 +1291,8 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
                             order, d_flags, recs, words);
                 break;
+            case QM_CLOSEST_VECTOR:
+                throw Error(SDFX_ERR_INVALID_ARGUMENT, "closest-point vectors are not an SDF query mode");
         }
         if (split)
             launch_coop(mode, d_reqs, PointReq{}, words, recs, (unsigned)std::min<uint64_t>(2048, (total + 3) / 4),
