@@ -81,6 +81,20 @@ struct ReqView {
         else return one;
     }
 };
+// A batch request that every active lane of the wave shares, read as constant
+// memory: the table is written before a launch and never during one, so its
+// fields come through the scalar cache (s_load) instead of one vector load per
+// field and lane.
+__device__ __forceinline__ GridReq wave_req(const GridReq* table, uint32_t q0) {
+    using ConstWord = const __attribute__((address_space(4))) uint32_t;
+    static_assert(sizeof(GridReq) % 4 == 0, "GridReq is read as 32-bit words");
+    ConstWord* w = (ConstWord*)(table + q0);
+    GridReq r;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(GridReq) / 4); ++i) d[i] = w[i];  // unused words are dropped
+    return r;
+}
 template <int MODE>
 constexpr int mode_sign = MODE & 1;
 template <int MODE>
@@ -599,13 +613,11 @@ struct PropAcc {
 // request q to its mesh-local triangle tri, kept if its key is below the
 // neighbour's current key.
 template <int MODE>
-__device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
-                                          uint64_t* keys, uint8_t* __restrict__ changed,
-                                          uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
-                                          DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
-                                          uint32_t q, uint64_t cur, PropAcc& acc) {
+__device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, uint8_t* __restrict__ changed,
+                                              uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
+                                              DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
+                                              uint32_t q, uint64_t cur, PropAcc& acc) {
     constexpr int SIGN = mode_sign<MODE>;
-    const GridReq& R = reqs[q];
     const TriRec tr = expand_leaf(R.ltris[tri]);  // three 16-byte loads instead of six, the same values
     const float px = cell_coord(R.g, 0, x), py = cell_coord(R.g, 1, y), pz = cell_coord(R.g, 2, z);
     float dx, dy, dz, cx, cy, cz;
@@ -642,6 +654,29 @@ __device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
         }
         if (mode_batch<MODE>) acc.req = q;
         acc.imin = min(acc.imin, (uint32_t)(key >> 32));
+    }
+}
+
+// prop_eval_req for lane's request q. In a batch the wave's lanes nearly always
+// share one request (items lie request by request); the wave then reads it
+// once through the scalar cache.
+template <int MODE>
+__device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
+                                          uint64_t* keys, uint8_t* __restrict__ changed,
+                                          uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
+                                          DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
+                                          uint32_t q, uint64_t cur, PropAcc& acc) {
+    if constexpr (mode_batch<MODE>) {
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(q);
+        if (__ballot(q != q0) == 0) {
+            const GridReq R = wave_req(reqs.table, q0);
+            prop_eval_req<MODE>(R, keys, changed, tile_changed, min_next, flags, x, y, z, tri, q0, cur, acc);
+        } else {
+            prop_eval_req<MODE>(reqs.table[q], keys, changed, tile_changed, min_next, flags, x, y, z, tri, q, cur,
+                                acc);
+        }
+    } else {
+        prop_eval_req<MODE>(reqs.one, keys, changed, tile_changed, min_next, flags, x, y, z, tri, 0u, cur, acc);
     }
 }
 
@@ -726,13 +761,26 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
             x = ix + k / 9u - 1u;
             y = iy + (k / 3u) % 3u - 1u;
             z = iz + k % 3u - 1u;
-            const GridReq& R = reqs[q];
-            if (x < R.g.n[0] && y < R.g.n[1] && z < R.g.n[2]) {  // -1 wraps to 2^32 - 1
+        }
+        // A neighbour already holding this triangle holds exactly the key the
+        // triangle would give it, and keys only decrease.
+        auto candidate = [&](const GridReq& R) {
+            if (j < j1 && x < R.g.n[0] && y < R.g.n[1] && z < R.g.n[2]) {  // -1 wraps to 2^32 - 1
                 cur = keys[R.cell_off + cell_index(R.g, x, y, z)];
-                // A neighbour already holding this triangle holds exactly the
-                // key the triangle would give it, and keys only decrease.
                 keep = cur == KEY_EMPTY || key_tri(cur, SIGN) != tri;
             }
+        };
+        if constexpr (mode_batch<MODE>) {
+            // lane 0 is in range whenever the loop runs
+            const uint32_t q0 = __builtin_amdgcn_readfirstlane(q);
+            if (__ballot(j < j1 && q != q0) == 0) {
+                const GridReq R = wave_req(reqs.table, q0);
+                candidate(R);
+            } else {
+                candidate(reqs.table[q]);
+            }
+        } else {
+            candidate(reqs.one);
         }
         const uint64_t m = __ballot(keep);
         if (keep) {
