@@ -81,24 +81,33 @@ struct ReqView {
         else return one;
     }
 };
-// A batch request that every active lane of the wave shares, read as constant
-// memory: the table is written before a launch and never during one, so its
-// fields come through the scalar cache (s_load) instead of one vector load per
-// field and lane.
+// A batch request that every active lane of the wave shares: the table is
+// written before a launch and never during one.
 __device__ __forceinline__ GridReq wave_req(const GridReq* table, uint32_t q0) {
-    using ConstWord = const __attribute__((address_space(4))) uint32_t;
-    static_assert(sizeof(GridReq) % 4 == 0, "GridReq is read as 32-bit words");
-    ConstWord* w = (ConstWord*)(table + q0);
-    GridReq r;
-    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(GridReq) / 4); ++i) d[i] = w[i];  // unused words are dropped
-    return r;
+    return wave_uniform_load(table, q0);
 }
 template <int MODE>
 constexpr int mode_sign = MODE & 1;
 template <int MODE>
 constexpr bool mode_batch = (MODE & 2) != 0;
+
+// Calls f(R, q) with request q of the lane. In a batch whose active lanes all
+// hold the same request (the usual case: work is laid out request by request)
+// the wave reads it once with wave_req; otherwise each lane reads its own.
+template <int MODE, class F>
+__device__ __forceinline__ void with_req(const ReqView<mode_batch<MODE>>& reqs, uint32_t q, F&& f) {
+    if constexpr (mode_batch<MODE>) {
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(q);
+        if (__ballot(q != q0) == 0) {
+            const GridReq R = wave_req(reqs.table, q0);
+            f(R, q0);
+        } else {
+            f(reqs.table[q], q);
+        }
+    } else {
+        f(reqs.one, 0u);
+    }
+}
 
 __device__ __forceinline__ float cell_coord(const GridDev& g, int k, uint32_t i) {
     return g.first[k] + (float)i * g.size[k];  // grid.rs:289-295
@@ -304,9 +313,13 @@ __global__ void __launch_bounds__(256) k_seed_boxes(uint64_t nt, ReqView<mode_ba
         counts[t] = 0;
         return;
     }
-    const GridDev g = reqs[q].g;
-    const TriRec tr = reqs[q].tris[t - reqs[q].tri_off];
-    reqs[q].ltris[t - reqs[q].tri_off] = make_leafrec(tr);
+    GridDev g;
+    TriRec tr;
+    with_req<MODE>(reqs, q, [&](const GridReq& R, uint32_t) {
+        g = R.g;
+        tr = R.tris[t - R.tri_off];
+        R.ltris[t - R.tri_off] = make_leafrec(tr);
+    });
     Box b;
     tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
     uint32_t mn[3], mx[3];
@@ -397,27 +410,29 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
             }
             const uint64_t t = t0 + lo;
             const uint64_t off_t = in_lds ? s_off[lo] : offsets[t];
-            const GridReq& R = reqs[mode_batch<MODE> ? tri_req[t] : 0u];
             const SeedBox sb = boxes[t];
-            uint64_t local = p - off_t;
-            uint64_t yz = (uint64_t)sb.ext[1] * sb.ext[2];
-            if (yz == 0 || local >= yz * sb.ext[0]) {  // cannot happen with a consistent prefix sum
-                atomicOr(&flags[R.flag].internal, 1u);
-                continue;
-            }
-            uint32_t x = sb.mn[0] + (uint32_t)(local / yz);
-            uint64_t r = local % yz;
-            uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
-            uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
-            const TriRec tr = R.tris[t - R.tri_off];
-            float d = cell_dist<SIGN>(R.g, x, y, z, tr);
-            n_evals++;
-            if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
-            if (key_valid(d)) {
-                uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
-                uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
-                if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
-            }
+            with_req<MODE>(reqs, mode_batch<MODE> ? tri_req[t] : 0u, [&](const GridReq& R, uint32_t) {
+                uint64_t local = p - off_t;
+                uint64_t yz = (uint64_t)sb.ext[1] * sb.ext[2];
+                if (yz == 0 || local >= yz * sb.ext[0]) {  // cannot happen with a consistent prefix sum
+                    atomicOr(&flags[R.flag].internal, 1u);
+                    return;
+                }
+                uint32_t x = sb.mn[0] + (uint32_t)(local / yz);
+                uint64_t r = local % yz;
+                uint32_t y = sb.mn[1] + (uint32_t)(r / sb.ext[2]);
+                uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
+                // the 48-byte copy k_seed_boxes wrote: three loads instead of six, the same values
+                const TriRec tr = expand_leaf(R.ltris[t - R.tri_off]);
+                float d = cell_dist<SIGN>(R.g, x, y, z, tr);
+                n_evals++;
+                if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
+                if (key_valid(d)) {
+                    uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
+                    uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
+                    if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+                }
+            });
         }
         __syncthreads();  // s_off and s_t0 / s_t1 are reused by the next chunk
     }
@@ -486,7 +501,11 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
     for (uint32_t tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
         if (!tile_changed[tile_id]) continue;  // block-uniform: no flagged cell in the tile
         const uint32_t q = mode_batch<MODE> ? tile_req[tile_id] : 0u;
-        const GridReq& R = reqs[q];
+        // the tile's request, the same in every lane: read through the scalar cache
+        const GridReq R = [&] {
+            if constexpr (mode_batch<MODE>) return wave_req(reqs.table, __builtin_amdgcn_readfirstlane(q));
+            else return reqs.one;
+        }();
         const uint32_t local0 = (uint32_t)((uint64_t)tile_id * COMPACT_TILE - R.cell_off);
         const uint32_t c0 = local0 + threadIdx.x * COMPACT_PER_LANE;  // first cell of this lane, request-local
         const uint64_t g0 = R.cell_off + c0;                           // the same cell in the batch
@@ -878,13 +897,14 @@ __global__ void __launch_bounds__(256) k_ray_scatter(uint64_t nt, ReqView<mode_b
     int axis = (int)(id % 3);
     const uint32_t q = mode_batch<MODE> ? tri_req[t] : 0u;
     if (q == UINT32_MAX) return;  // a triangle of no request in this run
-    const GridReq& R = reqs[q];
-    const TriRec tr = R.tris[t - R.tri_off];
-    Box b;
-    tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
-    if (axis == 0) scatter_axis<0>(tr, b, R.g, bx + R.row_off[0]);
-    else if (axis == 1) scatter_axis<1>(tr, b, R.g, by + R.row_off[1]);
-    else scatter_axis<2>(tr, b, R.g, bz + R.row_off[2]);
+    with_req<MODE>(reqs, q, [&](const GridReq& R, uint32_t) {
+        const TriRec tr = expand_leaf(R.ltris[t - R.tri_off]);
+        Box b;
+        tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
+        if (axis == 0) scatter_axis<0>(tr, b, R.g, bx + R.row_off[0]);
+        else if (axis == 1) scatter_axis<1>(tr, b, R.g, by + R.row_off[1]);
+        else scatter_axis<2>(tr, b, R.g, bz + R.row_off[2]);
+    });
 }
 
 // Per row: bit i becomes the XOR of flip bits i..n-1 (the parity of hits whose
@@ -941,22 +961,23 @@ __global__ void __launch_bounds__(256) k_grid_finish(const uint64_t* __restrict_
                                                      const uint32_t* __restrict__ pz) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const GridReq& R = reqs[mode_batch<MODE> ? tile_req[i / COMPACT_TILE] : 0u];
-        const uint64_t l = i - R.cell_off;
-        if (l >= R.ncells) continue;  // tile padding after the request's last cell
-        const GridDev& g = R.g;
-        constexpr int SIGN = mode_sign<MODE>;
-        float v = key_value(keys[i], SIGN);
-        if (SIGN == SDFX_SIGN_RAYCAST) {
-            uint32_t z = (uint32_t)(l % g.n[2]);
-            uint32_t y = (uint32_t)((l / g.n[2]) % g.n[1]);
-            uint32_t x = (uint32_t)(l / ((uint64_t)g.n[1] * g.n[2]));
-            uint32_t ox = (px[R.row_off[0] + ((uint64_t)y * g.n[2] + z) * g.words[0] + (x >> 5)] >> (x & 31u)) & 1u;
-            uint32_t oy = (py[R.row_off[1] + ((uint64_t)x * g.n[2] + z) * g.words[1] + (y >> 5)] >> (y & 31u)) & 1u;
-            uint32_t oz = (pz[R.row_off[2] + ((uint64_t)x * g.n[1] + y) * g.words[2] + (z >> 5)] >> (z & 31u)) & 1u;
-            if (ox + oy + oz >= 2u) v = -v;  // generate/grid.rs:633-638
-        }
-        R.out[l] = v;
+        with_req<MODE>(reqs, mode_batch<MODE> ? tile_req[i / COMPACT_TILE] : 0u, [&](const GridReq& R, uint32_t) {
+            const uint64_t l = i - R.cell_off;
+            if (l >= R.ncells) return;  // tile padding after the request's last cell
+            const GridDev& g = R.g;
+            constexpr int SIGN = mode_sign<MODE>;
+            float v = key_value(keys[i], SIGN);
+            if (SIGN == SDFX_SIGN_RAYCAST) {
+                uint32_t z = (uint32_t)(l % g.n[2]);
+                uint32_t y = (uint32_t)((l / g.n[2]) % g.n[1]);
+                uint32_t x = (uint32_t)(l / ((uint64_t)g.n[1] * g.n[2]));
+                uint32_t ox = (px[R.row_off[0] + ((uint64_t)y * g.n[2] + z) * g.words[0] + (x >> 5)] >> (x & 31u)) & 1u;
+                uint32_t oy = (py[R.row_off[1] + ((uint64_t)x * g.n[2] + z) * g.words[1] + (y >> 5)] >> (y & 31u)) & 1u;
+                uint32_t oz = (pz[R.row_off[2] + ((uint64_t)x * g.n[1] + y) * g.words[2] + (z >> 5)] >> (z & 31u)) & 1u;
+                if (ox + oy + oz >= 2u) v = -v;  // generate/grid.rs:633-638
+            }
+            R.out[l] = v;
+        });
     }
 }
 

@@ -182,6 +182,22 @@ void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method si
 // Adds v to out[blockIdx.x] once per block (block-wide sum in LDS). Each block
 // owns its slot, so the counters of consecutive launches on one stream need
 // no atomics; the host sums the slots.
+// Element i of a table that the running kernel does not write, with i the same
+// in every active lane: read as constant memory, so the loads go through the
+// scalar cache (s_load) instead of one vector load per word and lane. Words
+// the caller does not use are dropped by the compiler.
+template <class T>
+__device__ __forceinline__ T wave_uniform_load(const T* table, uint32_t i) {
+    using ConstWord = const __attribute__((address_space(4))) uint32_t;
+    static_assert(sizeof(T) % 4 == 0, "read as 32-bit words");
+    ConstWord* w = (ConstWord*)(table + i);
+    T r;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = w[k];
+    return r;
+}
+
 __device__ __forceinline__ void block_count(unsigned long long* out, uint32_t v) {
     __shared__ unsigned long long sdfx_block_sum;
     if (threadIdx.x == 0) sdfx_block_sum = 0;

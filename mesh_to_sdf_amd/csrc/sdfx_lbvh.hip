@@ -1215,10 +1215,20 @@ __global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __rest
     if (i < total) {  // no early return: the wave appends its deferred queries together
         const uint64_t g = order ? order[i] : i;
         const uint32_t r = point_req_of(reqs, nreq, g);
-        const PointReq& R = reqs[r];
         TravStats ts;
-        dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag, ts,
-                                R.budget, deferred ? &rec : nullptr);
+        auto run = [&](const PointReq& R) {
+            dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag,
+                                    ts, R.budget, deferred ? &rec : nullptr);
+        };
+        // queries lie request by request: a wave nearly always holds one, whose
+        // fields (tree pointers among them) then sit in scalar registers
+        const uint32_t r0 = __builtin_amdgcn_readfirstlane(r);
+        if (__ballot(r != r0) == 0) {
+            const PointReq R = wave_uniform_load(reqs, r0);
+            run(R);
+        } else {
+            run(reqs[r]);
+        }
         rec.req = r;
     }
     if (deferred) {  // kernel argument: uniform
