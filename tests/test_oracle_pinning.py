@@ -5,7 +5,6 @@ import math
 
 import numpy as np
 import pytest
-from hypothesis import given, settings, strategies as st
 
 import refcases as R
 
@@ -144,15 +143,22 @@ def _degenerate(a, b, c):
     return cmp(a, b) or cmp(a, c) or cmp(b, c)
 
 
-coord = st.floats(-10.0, 10.0, width=32, allow_nan=False)
-vec3 = st.lists(coord, min_size=3, max_size=3)
+def _uniform_cases(n, seed):
+    """n draws of (p, a, b, c), each coordinate uniform in [-10, 10) as FP32:
+    the distribution of the reference's proptest strategies
+    (prop::array::uniform3(-10.0f32..10.0), geo.rs:229-232, 263-266). The
+    properties are statistical (an adversarial search finds ill-conditioned
+    triangles beyond the 1e-3 tolerance), so the cases follow the reference's
+    distribution with a fixed seed."""
+    g = np.random.default_rng(seed)
+    return g.uniform(-10.0, 10.0, size=(n, 4, 3)).astype(np.float32)
 
 
-@settings(max_examples=1000, deadline=None)
-@given(vec3, vec3, vec3, vec3)
-def test_closest_point_vs_baseline(p, a, b, c):
+@pytest.mark.parametrize("case", range(1000))
+def test_closest_point_vs_baseline(case, _cases=_uniform_cases(1000, 229)):
     """geo.rs:229-255: Embree closest point vs the SDFGen-style baseline, ulps 5 / eps 1e-3."""
     import oracle as O
+    p, a, b, c = (list(map(float, v)) for v in _cases[case])
     if _degenerate(a, b, c):
         return
     d = O.point_triangle_distance(p, a, b, c)
@@ -170,21 +176,10 @@ def test_proptest_regressions(oracle, case):
     assert not np.isnan(d) and oracle.approx_eq_f32(d, base, 5, 1e-3)
 
 
-def _uniform_cases(n, seed):
-    """n draws of (p, a, b, c), each coordinate uniform in [-10, 10) as FP32:
-    the distribution of the reference's proptest strategy
-    (prop::array::uniform3(-10.0f32..10.0), geo.rs:263-266)."""
-    g = np.random.default_rng(seed)
-    return g.uniform(-10.0, 10.0, size=(n, 4, 3)).astype(np.float32)
-
-
 @pytest.mark.parametrize("case", range(1000))
 def test_aligned_ray_vs_generic(case, _ray_cases=_uniform_cases(1000, 262)):
     """geo.rs:262-287: the aligned test agrees with the generic test on all
-    three axes, on the reference's 1000 cases drawn as it draws them. (The
-    property is statistical: an adversarial search finds ill-conditioned
-    triangles where the two tests differ by more than the 1e-3 tolerance, so
-    the cases follow the reference's distribution with a fixed seed.)"""
+    three axes, on 1000 cases drawn as the reference draws them."""
     import oracle as O
     p, a, b, c = (list(map(float, v)) for v in _ray_cases[case])
     for axis, d in enumerate(([1, 0, 0], [0, 1, 0], [0, 0, 1])):
