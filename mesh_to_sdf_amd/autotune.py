@@ -14,6 +14,8 @@ Knobs:
   and keeps the largest pair count at which the exhaustive one is still faster.
 * grid_prop_blocks: blocks per grid propagation launch, timed on a 96^3 and a
   128^3 grid.
+* grid_batch_prop_blocks: the same for batched grid calls, timed on the first
+  32 requests of the batched workload.
 * lbvh_visit_budget: node visits after which an LBVH nearest search moves to
   the per-wave pass. 0 (the library's default) picks 15 x cbrt(triangles) per
   mesh; the tuner times it against fixed budgets with Bvh(Raycast) on the
@@ -137,6 +139,21 @@ def tune_grid_blocks(L, sp, log, candidates=(1024, 2048, 4096, 6144, 8192)):
     return min(total, key=total.get), {str(c): t * 1e3 for c, t in total.items()}
 
 
+def tune_batch_grid_blocks(L, sp, log, candidates=(1024, 2048, 3072, 4096)):
+    """grid_batch_prop_blocks, timed on the first 32 requests of the batched
+    workload (one generate_grid_sdf_batch call each; the host copies are the
+    same for every candidate)."""
+    import mesh_to_sdf_amd as sdf
+    reqs = [(v, sdf.Topology.TriangleList(f), sdf.Grid(first, size, count))
+            for v, f, first, size, count in S.config4(n_meshes=32)]
+    ms = {}
+    for c in candidates:
+        with _lib.tuned("grid_batch_prop_blocks", c):
+            ms[c] = _timed(lambda: sdf.generate_grid_sdf_batch(reqs, sdf.SignMethod.Raycast), 3)
+        log(f"  batch of {len(reqs)} grids, {c} blocks: {ms[c] * 1e3:.3f} ms")
+    return min(ms, key=ms.get), {str(c): t * 1e3 for c, t in ms.items()}
+
+
 def tune(out_path: str = DEFAULT_PATH, log=print) -> dict:
     import torch
     torch.cuda.set_device(0)
@@ -148,12 +165,14 @@ def tune(out_path: str = DEFAULT_PATH, log=print) -> dict:
     budget, budget_ms = tune_visit_budget(L, sp, log)
     log("grid_prop_blocks:")
     blocks, grid_ms = tune_grid_blocks(L, sp, log)
+    log("grid_batch_prop_blocks:")
+    batch_blocks, batch_ms = tune_batch_grid_blocks(L, sp, log)
     result = {
         "device": torch.cuda.get_device_name(0),
         "knobs": {"exhaustive_max_pairs": int(crossover), "grid_prop_blocks": int(blocks),
-                  "lbvh_visit_budget": int(budget)},
+                  "lbvh_visit_budget": int(budget), "grid_batch_prop_blocks": int(batch_blocks)},
         "measurements": {"exhaustive_vs_lbvh": rows, "lbvh_ms_by_visit_budget": budget_ms,
-                         "grid_ms_by_blocks": grid_ms},
+                         "grid_ms_by_blocks": grid_ms, "batch_grid_ms_by_blocks": batch_ms},
     }
     if out_path:
         with open(out_path, "w") as fh:

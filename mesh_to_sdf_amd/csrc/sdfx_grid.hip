@@ -1193,10 +1193,13 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     uint64_t* total = A.alloc<uint64_t>(1);
     SDFX_LAUNCH(k_seed_boxes<MODE>, dim3(blocks_for(nt, 256)), dim3(256), 0, s, nt, rv, tri_req, boxes, counts);
     exclusive_scan_u64(counts, counts, nt, total, A);
-    // distance-evaluation counters, only while kernel timing is on
-    // per-block slots: seeding (8192 blocks), then propagation (4096 blocks)
+    // a batch's rounds are large: one resident generation of blocks does best there
+    const int64_t blocks_knob = mode_batch<MODE> ? tuning().grid_batch_prop_blocks : tuning().grid_prop_blocks;
+    const unsigned prop_blocks = (unsigned)std::max<int64_t>(XCDS, blocks_knob) / XCDS * XCDS;
+    // distance-evaluation counters, only while kernel timing is on: one slot
+    // per block of k_seed_pairs (8192 blocks), then one per propagation block
     unsigned long long* evals = nullptr;
-    const uint64_t seed_slots = 8192, prop_slots = 4096;
+    const uint64_t seed_slots = 8192, prop_slots = prop_blocks;
     if (profile_enabled()) {
         evals = A.alloc<unsigned long long>(seed_slots + prop_slots);
         fill_u32(reinterpret_cast<uint32_t*>(evals), 2 * (seed_slots + prop_slots), 0u, s);
@@ -1218,7 +1221,6 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     SDFX_LAUNCH(k_mark_seeded<MODE>, dim3(stride_blocks(ncells)), dim3(256), 0, s, keys, ncells, tile_req, changed,
                 tile_changed, minw[0]);
     SDFX_HIP(hipGetLastError());
-    const unsigned prop_blocks = (unsigned)std::max<int64_t>(XCDS, tuning().grid_prop_blocks) / XCDS * XCDS;
     // one block per flagged-tile scan, up to 16,384 blocks (a block walks the tiles
     // beyond that one after another; batched grids: 4,096 -> 16,384 cut compaction 9%)
     const unsigned compact_blocks = std::min<uint32_t>(ntiles, 16384u);

@@ -145,6 +145,7 @@ struct Tuning {
     int64_t exhaustive_max_pairs;
     int64_t grid_prop_blocks;
     int64_t lbvh_visit_budget;  // node visits after which a nearest search moves to the per-wave pass (0: by mesh size)
+    int64_t grid_batch_prop_blocks;  // grid_prop_blocks for launches that serve several requests
 };
 Tuning& tuning();
 

@@ -206,7 +206,7 @@ namespace sdfx {
 Tuning& tuning() {
     // 2^26 pairs: about 10k queries against 6k triangles, where the exhaustive
     // kernel finishes well before an LBVH build and traversal would.
-    static Tuning t{int64_t(1) << 26, 4096, 0};
+    static Tuning t{int64_t(1) << 26, 4096, 0, 2048};
     return t;
 }
 
@@ -324,6 +324,7 @@ int64_t* tuning_slot(const char* key) {
     if (k == "exhaustive_max_pairs") return &sdfx::tuning().exhaustive_max_pairs;
     if (k == "grid_prop_blocks") return &sdfx::tuning().grid_prop_blocks;
     if (k == "lbvh_visit_budget") return &sdfx::tuning().lbvh_visit_budget;
+    if (k == "grid_batch_prop_blocks") return &sdfx::tuning().grid_batch_prop_blocks;
     return nullptr;
 }
 }  // namespace
@@ -331,7 +332,9 @@ int64_t* tuning_slot(const char* key) {
 int sdfx_set_tuning(const char* key, int64_t value) {
     int64_t* slot = tuning_slot(key);
     if (!slot) return -1;
-    if (std::string(key) == "grid_prop_blocks" && (value < 8 || value > 65536)) return -1;
+    if ((std::string(key) == "grid_prop_blocks" || std::string(key) == "grid_batch_prop_blocks") &&
+        (value < 8 || value > 65536))
+        return -1;
     if (std::string(key) == "lbvh_visit_budget" && value != 0 && (value < 16 || value > INT32_MAX)) return -1;
     *slot = value;
     return 0;
