@@ -1255,13 +1255,20 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     BatchProbe probe;
     int cur = 0;
     uint64_t round = 0;
+    // Once the front is thin (the last reported round took fewer than 2,048
+    // cells: at most 53,248 item-neighbour pairs), 256 blocks cover every pair
+    // in one pass and the other blocks would only cost dispatch time. The
+    // propagation's loops are grid-strided, so a round that grows again is
+    // still complete.
+    uint32_t last_taken = UINT32_MAX;
     auto enqueue_batch = [&](int slot) {
         const uint32_t seq = probe.arm(slot);
+        const unsigned blocks = last_taken < 2048u ? std::min(prop_blocks, 256u) : prop_blocks;
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
                         taken[cur ^ 1], taken[cur], released, (int)SIGN);
-            SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(prop_blocks), dim3(256), 0, s, items, region, ctr[cur],
+            SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(blocks), dim3(256), 0, s, items, region, ctr[cur],
                         rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
                         totals + cur, b == batch - 1 ? probe.probe(slot) : nullptr, seq, d_flags,
                         evals ? evals + seed_slots : nullptr);
@@ -1274,6 +1281,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         if (round > max_rounds) throw Error(SDFX_ERR_DEVICE, "grid propagation did not converge");
         enqueue_batch(slot ^ 1);
         const uint32_t taken_cells = probe.wait(slot, s);
+        last_taken = taken_cells;
         if (taken_cells == 0) break;  // the batch after it only runs empty rounds
         batch = taken_cells < 512 ? 2 : (taken_cells < 8192 ? 4 : 8);
     }
