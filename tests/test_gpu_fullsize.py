@@ -7,7 +7,8 @@ do not need the oracle to run at that size:
   algorithm that evaluates every triangle for every query, and every answer is
   finite;
 * the batched grid workload (256 meshes, 84.3M cells): every request of one
-  batched call gets the same bits as the same request alone.
+  batched call gets the same bits as the same request alone;
+* a single 17.3M-cell grid: the same bits alone and inside a batch.
 The oracle pins both paths at smaller sizes (test_gpu_parity.py,
 test_gpu_grid_batch.py)."""
 import numpy as np
@@ -61,3 +62,21 @@ def test_noise_sphere_4m_closest_vectors_finite_and_consistent(sdf):
     assert np.array_equal(inside.astype(bool), d < 0)
     assert np.allclose(np.linalg.norm(vec.astype(np.float64), axis=1), np.abs(d), rtol=1e-5, atol=1e-7)
     acc.close()
+
+
+def test_large_single_grid_equals_batched(sdf):
+    """A single grid above 4,096 tiles of 4,096 cells (264 x 256 x 256 = 17.3M
+    cells), where the compaction reads cell flags only for flagged tiles: the
+    same bits as the request inside a batch (next to a small second request),
+    whose kernels address requests through the batch table instead."""
+    v, f, _, _, _ = S.config2()
+    first, size, count = S.grid_for(v, (264, 256, 256))
+    assert count[0] * count[1] * count[2] > 4096 * 4096
+    T = sdf.Topology.TriangleList(f)
+    grid = sdf.Grid(first, size, count)
+    alone = sdf.generate_grid_sdf(v, T, grid, sdf.SignMethod.Raycast)
+    small = sdf.Grid(*S.grid_for(v, (20, 20, 20)))
+    batch = sdf.generate_grid_sdf_batch([(v, T, grid), (v, T, small)], sdf.SignMethod.Raycast)
+    assert np.all(np.isfinite(alone))
+    assert np.array_equal(bits(batch[0]), bits(alone))
+    assert np.array_equal(bits(batch[1]), bits(sdf.generate_grid_sdf(v, T, small, sdf.SignMethod.Raycast)))
