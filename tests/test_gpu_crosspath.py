@@ -87,17 +87,23 @@ def test_device_grid_continuity(sdf):
 
 
 def test_results_independent_of_launch_shapes(sdf):
-    """Every tuning knob changes launch shapes only: the grid with different
-    propagation grid sizes, and the point paths exhaustive or through the
+    """Every tuning knob changes launch shapes only: the grid, alone and
+    batched, with different propagation grid sizes, and the point paths exhaustive or through the
     LBVH (with any visit budget for the per-query pass), give the same bits."""
     v, f, first, size, count = S.config2()
     grid = sdf.Grid(first, size, (64, 64, 64))
     T = sdf.Topology.TriangleList(f)
     outs = []
-    for blocks in (256, 4096, 9000):
+    for blocks in (8, 256, 4096, 9000):  # 8: each block strides over many pairs
         with sdf._lib.tuned("grid_prop_blocks", blocks):
             outs.append(sdf.generate_grid_sdf(v, T, grid, sdf.SignMethod.Raycast))
     assert all(np.array_equal(bits(outs[0]), bits(o)) for o in outs[1:])
+    reqs = [(v, T, grid), (v, T, sdf.Grid(*S.grid_for(v, (24, 30, 18))))]
+    for blocks in (8, 2048, 16384):
+        with sdf._lib.tuned("grid_batch_prop_blocks", blocks):
+            batch = sdf.generate_grid_sdf_batch(reqs, sdf.SignMethod.Raycast)
+        assert np.array_equal(bits(batch[0]), bits(outs[0])), blocks
+        assert np.array_equal(bits(batch[1]), bits(sdf.generate_grid_sdf(*reqs[1], sdf.SignMethod.Raycast))), blocks
     q = S.uniform_queries(v, 20000, 77)
     for m in (sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast), sdf.AccelerationMethod.Rtree,
               sdf.AccelerationMethod.RtreeBvh):
