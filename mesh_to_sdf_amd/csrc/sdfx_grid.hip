@@ -528,9 +528,10 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
         }();
         const uint32_t local0 = (uint32_t)((uint64_t)tile_id * COMPACT_TILE - R.cell_off);
         const uint32_t c0 = local0 + threadIdx.x * COMPACT_PER_LANE;  // first cell of this lane, request-local
-        // this lane's 16 cell flags (whole tiles are allocated), and the
-        // request's band words (read by one wave, shared through LDS), are read together
+        // this lane's 16 cell flags (whole tiles are allocated), unless read above
         if (!early_flags) v = *reinterpret_cast<const uint4*>(changed + g0);
+        // the request's band words, shared through LDS (already loaded above for
+        // a small single grid: then these are LDS writes only)
         if (wave == 0) {
             const uint32_t m = early_words ? m0 : req_min<MODE>(min_cur, q);
             const uint32_t p = early_words ? p0 : req_sum<MODE>(taken_prev, q);
@@ -540,6 +541,19 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
                 far_released = early_words ? r0 : released[q];
             }
         }
+        uint32_t flagged = 0;  // bit k: cell c0 + k is flagged
+        {
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < COMPACT_PER_LANE; ++k) flagged |= ((w[k >> 2] >> (8 * (k & 3))) & 1u) << k;
+            if (c0 >= R.ncells) flagged = 0;
+            else if (R.ncells - c0 < (uint32_t)COMPACT_PER_LANE) flagged &= (1u << (R.ncells - c0)) - 1u;
+        }
+        // every flagged cell's key, issued together and before the barrier
+        // below, so they are in flight while the band words are shared
+        uint64_t kv[COMPACT_PER_LANE];
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER_LANE; ++k) kv[k] = ((flagged >> k) & 1u) ? keys[g0 + k] : KEY_EMPTY;
         __syncthreads();
         const float mf = bitsf(band_min);
         const bool thin = prev_taken < R.release;
@@ -550,18 +564,6 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
         // a NaN band (or no flagged cell), a thin previous front, or the far field: every flagged cell
         if (!(limit >= mf) || thin || far_released) limit = INFINITY;
         const uint32_t lbits = fbits(limit);  // non-negative floats order as their bits
-        uint32_t flagged = 0;  // bit k: cell c0 + k is flagged
-        {
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int k = 0; k < COMPACT_PER_LANE; ++k) flagged |= ((w[k >> 2] >> (8 * (k & 3))) & 1u) << k;
-            if (c0 >= R.ncells) flagged = 0;
-            else if (R.ncells - c0 < (uint32_t)COMPACT_PER_LANE) flagged &= (1u << (R.ncells - c0)) - 1u;
-        }
-        // every flagged cell's key first, so the loads are in flight together
-        uint64_t kv[COMPACT_PER_LANE];
-#pragma unroll
-        for (int k = 0; k < COMPACT_PER_LANE; ++k) kv[k] = ((flagged >> k) & 1u) ? keys[g0 + k] : KEY_EMPTY;
         uint32_t take = 0, dmin = UINT32_MAX;
 #pragma unroll
         for (int k = 0; k < COMPACT_PER_LANE; ++k) {
