@@ -2,11 +2,17 @@
 // for one request or a batch of independent (mesh, grid) requests run through
 // the same launches.
 //
-// Cells hold a 64-bit key ordered like the reference's stored distance:
+// Cells hold a 64-bit key:
 //   Raycast: bits(d) << 32 | triangle        (d >= 0, so the bits order as floats)
 //   Normal:  bits(|d|) << 32 | negative << 31 | triangle
-// so one atomicMin keeps the smallest distance, then (Normal) the positive
-// one, then the lowest triangle index (the index within the request's mesh).
+// and keep the minimum under the order in which the reference replaces a
+// stored value (compare_distances, lib.rs:315-332), the lower triangle (the
+// index within the request's mesh) first among equal distances. For Raycast
+// that is the keys' integer order (one atomicMin). In Normal mode it is the
+// integer order except that a positive value goes before a negative one it
+// approximately equals (key_before); a compare-and-swap loop keeps it. That
+// order is a strict total order, so the minimum does not depend on the order
+// in which candidates arrive.
 //
 // 1. Seeding (generate/grid.rs:383-457): every (triangle, cell) pair of each
 //    triangle's snapped, widened padded box, one lane per pair (the pair list
@@ -130,15 +136,44 @@ __device__ __forceinline__ uint32_t snap_axis(float c, uint32_t n) {
     return (uint32_t)(i < 0 ? 0 : (i > hi ? hi : i));
 }
 
+// compare_distances(d, f32::MAX) is Less: |d| below f32::MAX (not NaN), and
+// for a negative d not within 2 ulps of it (those approximately equal the
+// positive f32::MAX and come after it).
 __device__ __forceinline__ bool key_valid(float d) {
-    float a = fabsf(d);
-    return a == a && fbits(a) < 0x7f7ffffdu;  // compare_distances(d, f32::MAX) is Less
+    const float a = fabsf(d);
+    return a < FLT_MAX && ((int32_t)fbits(d) >= 0 || fbits(a) < 0x7f7ffffdu);
 }
 
 template <int SIGN>
 __device__ __forceinline__ uint64_t make_key(float d, uint32_t tri) {
     if (SIGN == SDFX_SIGN_RAYCAST) return ((uint64_t)fbits(d) << 32) | tri;
     return ((uint64_t)fbits(fabsf(d)) << 32) | ((uint64_t)(fbits(d) >> 31) << 31) | tri;
+}
+
+// Whether key a goes strictly before key b (see the top of the file). Valid
+// keys all go before KEY_EMPTY (an empty cell holds f32::MAX).
+template <int SIGN>
+__device__ __forceinline__ bool key_before(uint64_t a, uint64_t b) {
+    if (SIGN == SDFX_SIGN_NORMAL && ((a ^ b) >> 31 & 1ull) && a != KEY_EMPTY && b != KEY_EMPTY &&
+        approx_eq(bitsf((uint32_t)(a >> 32)), bitsf((uint32_t)(b >> 32))))
+        return ((b >> 31) & 1ull) != 0;  // an approximate tie of opposite signs: the positive one first
+    return a < b;
+}
+
+// Lowers *slot to key when key goes before its value (seen: a recent read of
+// it); returns whether it did. Values only ever move earlier in the order, so
+// a stale `seen` costs at most a failed compare-and-swap.
+template <int SIGN>
+__device__ __forceinline__ bool key_lower(uint64_t* slot, uint64_t key, uint64_t seen) {
+    if (!key_before<SIGN>(key, seen)) return false;
+    if (SIGN == SDFX_SIGN_RAYCAST) return key < atomicMin((unsigned long long*)slot, (unsigned long long)key);
+    uint64_t old = seen;
+    for (;;) {
+        const uint64_t prev = atomicCAS((unsigned long long*)slot, (unsigned long long)old, (unsigned long long)key);
+        if (prev == old) return true;
+        old = prev;
+        if (!key_before<SIGN>(key, old)) return false;
+    }
 }
 
 __device__ __forceinline__ uint32_t key_tri(uint64_t k, int sign) {
@@ -430,7 +465,7 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
                 if (key_valid(d)) {
                     uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
                     uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
-                    if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+                    key_lower<SIGN>(slot, key, *slot);
                 }
             });
         }
@@ -668,8 +703,13 @@ __device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, 
         // The key holds |d| = c. sqrt is monotone and correctly rounded, so
         // d2 >= (next float above c)^2 gives |d| > c: no improvement, and the
         // square root and sign can be skipped. (c is kept in the range where
-        // these squares are normal floats.)
-        const uint32_t cb = (uint32_t)(cur >> 32);
+        // these squares are normal floats.) In Normal mode a positive value
+        // still goes before a negative c it approximately equals: within
+        // 1e-6 (plus the rounding of the difference) or 2 ulps above it, so
+        // the bound is raised past both.
+        uint32_t cb = (uint32_t)(cur >> 32);
+        if (SIGN == SDFX_SIGN_NORMAL && ((cur >> 31) & 1ull))
+            cb = max(fbits(bitsf(cb) + 2e-6f), cb + 2u);
         if (cb >= 0x26800000u && cb < 0x5f000000u) {  // 2^-50 <= c < 2^63
             const float cp = bitsf(cb + 1u);
             const float lim = bitsf(fbits(cp * cp) + 1u);  // >= the exact square of cp
@@ -682,10 +722,8 @@ __device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, 
     if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
     if (!key_valid(d)) return;
     const uint64_t key = make_key<SIGN>(d, tri);
-    if (key >= cur) return;
     const uint64_t ci = R.cell_off + cell_index(R.g, x, y, z);
-    const uint64_t old = atomicMin((unsigned long long*)&keys[ci], (unsigned long long)key);
-    if (key < old) {
+    if (key_lower<SIGN>(&keys[ci], key, cur)) {
         changed[ci] = 1;
         tile_changed[ci / COMPACT_TILE] = 1;
         if (mode_batch<MODE> && q != acc.req && acc.imin != UINT32_MAX) {  // the lane moved on to another request

@@ -30,6 +30,8 @@ TOPO_LIST, TOPO_STRIP = 0, 1
 SIGN_RAYCAST, SIGN_NORMAL = 0, 1
 ACCEL_NONE, ACCEL_BVH, ACCEL_RTREE, ACCEL_RTREE_BVH = 0, 1, 2, 3
 GRID_SEQUENTIAL, GRID_THREADED, GRID_FRONTIER = 0, 1, 2
+FAULT_NONE, FAULT_SKIP_DIRECTION, FAULT_DROP_DEFERRED, FAULT_PACKED_ORDER = 0, 1, 2, 3
+NO_TRIANGLE = 0xFFFFFFFF
 
 _f = ctypes.POINTER(ctypes.c_float)
 _u32 = ctypes.POINTER(ctypes.c_uint32)
@@ -87,6 +89,13 @@ def lib() -> ctypes.CDLL:
             "orc_generate_grid_sdf": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64, ctypes.c_int, _f, _f,
                                                      _u64, ctypes.c_int, _f, ctypes.c_int, ctypes.c_int, _f,
                                                      ctypes.POINTER(GridStats)]),
+            "orc_generate_grid_sdf_ex": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64, ctypes.c_int, _f,
+                                                        _f, _u64, ctypes.c_int, _f, _u32, ctypes.c_int, ctypes.c_int,
+                                                        _f, ctypes.POINTER(GridStats)]),
+            "orc_grid_fixpoint_check": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64, ctypes.c_int, _f,
+                                                       _f, _u64, ctypes.c_int, _f, _u32, ctypes.c_int, _u64]),
+            "orc_set_fault": (None, [ctypes.c_int]),
+            "orc_nearest_bvh": (None, [_f, _u32, ctypes.c_uint64, _f, ctypes.c_uint64, _f, _u32, ctypes.c_int]),
             "orc_query_vector_to_closest_point": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64,
                                                                  ctypes.c_int, _f, ctypes.c_uint64, _f, _u8,
                                                                  ctypes.c_int]),
@@ -257,21 +266,50 @@ def generate_sdf(vertices, indices, topology, queries, accel, sign=SIGN_RAYCAST,
 
 
 def generate_grid_sdf(vertices, indices, topology, first_cell, cell_size, cell_count, sign=SIGN_RAYCAST,
-                      n_heaps=1, schedule=GRID_SEQUENTIAL, want_seed=False):
-    """Returns (status, distances, stats[, seeded preheap distances])."""
+                      n_heaps=1, schedule=GRID_SEQUENTIAL, want_seed=False, want_tri=False):
+    """Returns (status, distances, stats[, seeded preheap distances][, per-cell triangle])."""
     v, idx = _mesh_args(vertices, indices)
     cnt = np.ascontiguousarray(cell_count, dtype=np.uint64)
     ncells = int(cnt[0]) * int(cnt[1]) * int(cnt[2])
     out = np.zeros(ncells, np.float32)
     seed = np.zeros(ncells, np.float32) if want_seed else None
+    tri = np.zeros(ncells, np.uint32) if want_tri else None
     st = GridStats()
     (F, pf), (S, ps) = _ptr3(first_cell), _ptr3(cell_size)
-    s = lib().orc_generate_grid_sdf(_fp(v), len(v), None if idx is None else idx.ctypes.data_as(_u32),
-                                    0 if idx is None else idx.size, topology, pf, ps, cnt.ctypes.data_as(_u64),
-                                    sign, _fp(out), n_heaps, schedule, _fp(seed), ctypes.byref(st))
+    s = lib().orc_generate_grid_sdf_ex(_fp(v), len(v), None if idx is None else idx.ctypes.data_as(_u32),
+                                       0 if idx is None else idx.size, topology, pf, ps, cnt.ctypes.data_as(_u64),
+                                       sign, _fp(out), None if tri is None else tri.ctypes.data_as(_u32), n_heaps,
+                                       schedule, _fp(seed), ctypes.byref(st))
+    res = [s, out, st]
     if want_seed:
-        return s, out, st, seed
-    return s, out, st
+        res.append(seed)
+    if want_tri:
+        res.append(tri)
+    return tuple(res)
+
+
+def grid_fixpoint_check(vertices, indices, topology, first_cell, cell_size, cell_count, sign, values, tri_of,
+                        n_threads=None):
+    """Violations of the propagation's fixpoint (orc_grid_fixpoint_check):
+    dict with value_mismatch, not_fixpoint, no_triangle, no_triangle_value."""
+    v, idx = _mesh_args(vertices, indices)
+    cnt = np.ascontiguousarray(cell_count, dtype=np.uint64)
+    vals = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+    tri = np.ascontiguousarray(tri_of, dtype=np.uint32).reshape(-1)
+    assert vals.size == tri.size == int(cnt[0]) * int(cnt[1]) * int(cnt[2])
+    counts = np.zeros(4, np.uint64)
+    (F, pf), (S, ps) = _ptr3(first_cell), _ptr3(cell_size)
+    s = lib().orc_grid_fixpoint_check(_fp(v), len(v), None if idx is None else idx.ctypes.data_as(_u32),
+                                      0 if idx is None else idx.size, topology, pf, ps, cnt.ctypes.data_as(_u64),
+                                      sign, _fp(vals), tri.ctypes.data_as(_u32), n_threads or threads(),
+                                      counts.ctypes.data_as(_u64))
+    assert s == OK, s
+    return dict(zip(("value_mismatch", "not_fixpoint", "no_triangle", "no_triangle_value"), map(int, counts)))
+
+
+def set_fault(kind):
+    """Test-only: inject a fault into the GRID_FRONTIER schedule (FAULT_*)."""
+    lib().orc_set_fault(kind)
 
 
 def query_vector_to_closest_point(vertices, indices, topology, queries, n_threads=None):
@@ -294,6 +332,28 @@ def nearest_bruteforce(vertices, tris, points, n_threads=None):
     lib().orc_nearest_bruteforce(_fp(v), t.ctypes.data_as(_u32), len(t), _fp(p), len(p), _fp(d2),
                                  ti.ctypes.data_as(_u32), n_threads or threads())
     return d2, ti
+
+
+def nearest_bvh(vertices, tris, points, n_threads=None):
+    """nearest_bruteforce's (d2, triangle) through the oracle's BVH."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    d2 = np.zeros(len(p), np.float32)
+    ti = np.zeros(len(p), np.uint32)
+    lib().orc_nearest_bvh(_fp(v), t.ctypes.data_as(_u32), len(t), _fp(p), len(p), _fp(d2),
+                          ti.ctypes.data_as(_u32), n_threads or threads())
+    return d2, ti
+
+
+def grid_centres(first_cell, cell_size, cell_count):
+    """Cell centres first + i * size in FP32 with no FMA (grid.rs:289-295), z fastest."""
+    f = _v3(first_cell)
+    sz = _v3(cell_size)
+    n = [int(c) for c in cell_count]
+    ax = [f[k] + np.arange(n[k], dtype=np.float32) * sz[k] for k in range(3)]
+    X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(np.float32)
 
 
 def fp64_distance(vertices, tris, points, n_threads=None):

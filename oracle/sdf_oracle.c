@@ -952,7 +952,7 @@ static uint64_t seed_triangle(const ogrid* g, const float* verts, const uint32_t
 
 /* generate/grid.rs:495-558 for one heap */
 static int propagate(const ogrid* g, const float* verts, const uint32_t* tris, oheap* h, float* dist_arr,
-                     int sign, uint8_t* locks, uint64_t* pops) {
+                     uint32_t* dist_tri, int sign, uint8_t* locks, uint64_t* pops) {
     ostate s;
     uint64_t np = 0;
     while (heap_pop(h, &s)) {
@@ -971,6 +971,7 @@ static int propagate(const ogrid* g, const float* verts, const uint32_t* tris, o
                     if (locks) cell_lock(locks, ci);
                     if (orc_compare_distances(d, dist_arr[ci]) == -1) {
                         dist_arr[ci] = d;
+                        if (dist_tri) dist_tri[ci] = s.tri; /* the triangle the pushed state carries */
                         improved = 1;
                     }
                     if (locks) cell_unlock(locks, ci);
@@ -989,12 +990,14 @@ static int propagate(const ogrid* g, const float* verts, const uint32_t* tris, o
 static const uint64_t KEY_EMPTY = UINT64_MAX;
 
 /* A candidate is stored only where compare_distances(d, f32::MAX) is Less
- * (grid.rs:121, 446): |d| must not be NaN nor within 2 ulps of f32::MAX. */
+ * (grid.rs:121, 446): |d| below f32::MAX and not NaN, and for a negative d not
+ * within 2 ulps of f32::MAX either (those approximately equal f32::MAX, which
+ * is positive, so they come after it). */
 static inline int key_valid(float d) {
     float a = fabsf(d);
     uint32_t bits;
     memcpy(&bits, &a, 4);
-    return a == a && bits < 0x7f7ffffdu;
+    return a < FLT_MAX && (!sign_negative(d) || bits < 0x7f7ffffdu);
 }
 /* Raycast: (bits(d) << 32 | triangle); Normal: (bits(|d|) << 32 | negative << 31 | triangle) */
 static inline uint64_t make_key(float d, uint32_t t, int sign) {
@@ -1015,12 +1018,40 @@ static inline float key_value(uint64_t k, int sign) {
 static inline uint32_t key_tri(uint64_t k, int sign) {
     return sign == ORC_SIGN_NORMAL ? (uint32_t)(k & 0x7fffffffu) : (uint32_t)k;
 }
-static inline int atomic_min_u64(uint64_t* p, uint64_t v) {
+/* Whether key a comes strictly before key b in the order the reference keeps
+ * (lib.rs:315-332 compare_distances, "Less" replaces the stored value), with
+ * the lower triangle first among equal distances. Raycast values are never
+ * negative, so compare_distances is plain < on them and the packed key's
+ * integer order is that order. In Normal mode compare_distances orders
+ * values by |d| except that a positive value comes before a negative one it
+ * approximately equals (float-cmp's approx_eq, ulps 2, epsilon 1e-6); it is a
+ * strict weak order (the positive values it places before a negative |d| = y
+ * are those up to a threshold that grows with y), so this is a strict total
+ * order and the minimum under it does not depend on the order the candidates
+ * arrive in. An empty cell (f32::MAX) comes after every valid key. */
+static int g_fault;
+static inline int key_before(uint64_t a, uint64_t b, int sign) {
+    if (g_fault == ORC_FAULT_PACKED_ORDER) return a < b;
+    if (sign == ORC_SIGN_NORMAL && a != KEY_EMPTY && b != KEY_EMPTY && ((a ^ b) >> 31 & 1u)) {
+        uint32_t ab = (uint32_t)(a >> 32), bb = (uint32_t)(b >> 32);
+        float fa, fb;
+        memcpy(&fa, &ab, 4);
+        memcpy(&fb, &bb, 4);
+        if (orc_approx_eq_f32(fa, fb, 2, 1e-6f)) return (int)((b >> 31) & 1u); /* the positive one first */
+    }
+    return a < b;
+}
+/* Lowers *p to v when v comes before it (key_before); returns 1 if it did. */
+static inline int key_lower(uint64_t* p, uint64_t v, int sign) {
     uint64_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
-    while (v < cur)
+    while (key_before(v, cur, sign))
         if (__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return 1;
     return 0;
 }
+
+/* Test-only fault injection into the frontier schedule (orc_set_fault), so the
+ * tests can show that the fixpoint check catches a schedule that is wrong. */
+void orc_set_fault(int kind) { g_fault = kind; }
 
 static void frontier_seed_triangle(const ogrid* g, const float* verts, const uint32_t* tris, uint32_t t, int sign,
                                    uint64_t* keys, int* nan_seen) {
@@ -1041,13 +1072,14 @@ static void frontier_seed_triangle(const ogrid* g, const float* verts, const uin
                 uint64_t cell[3] = {x, y, z};
                 float d = cell_distance(g, verts, tris, t, cell, sign);
                 if (d != d) *nan_seen = 1;
-                if (key_valid(d)) atomic_min_u64(&keys[cell_idx(g, x, y, z)], make_key(d, t, sign));
+                if (key_valid(d)) key_lower(&keys[cell_idx(g, x, y, z)], make_key(d, t, sign), sign);
             }
 }
 
 /* Returns the number of rounds, or -1 on a NaN distance (the reference panics). */
 static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint32_t* tris, uint64_t nt, int sign,
-                                 float* dist_arr, float* seed_out, int n_threads, uint64_t* evals) {
+                                 float* dist_arr, uint32_t* tri_out, float* seed_out, int n_threads,
+                                 uint64_t* evals) {
     uint64_t ncells = g->n[0] * g->n[1] * g->n[2];
     uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (ncells + 1));
     uint8_t* changed = (uint8_t*)calloc(ncells + 1, 1);
@@ -1087,7 +1119,10 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
             uint32_t b = (uint32_t)(keys[i] >> 32);
             float v;
             memcpy(&v, &b, 4);
-            if (!(v <= limit)) continue;
+            if (!(v <= limit)) {
+                if (g_fault == ORC_FAULT_DROP_DEFERRED) changed[i] = 0;
+                continue;
+            }
             changed[i] = 0;
             fr_cell[nf] = i;
             fr_tri[nf] = key_tri(keys[i], sign);
@@ -1105,6 +1140,7 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dz = -1; dz <= 1; ++dz) {
                         if (!dx && !dy && !dz) continue; /* the cell itself can never improve */
+                        if (g_fault == ORC_FAULT_SKIP_DIRECTION && dx == 1 && !dy && !dz) continue;
                         int64_t x = cx + dx, y = cy + dy, z = cz + dz;
                         if (x < 0 || y < 0 || z < 0 || x >= (int64_t)g->n[0] || y >= (int64_t)g->n[1] ||
                             z >= (int64_t)g->n[2])
@@ -1114,12 +1150,14 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
                         n_evals++;
                         if (d != d) nan_seen = 1;
                         uint64_t ni = cell_idx(g, cell[0], cell[1], cell[2]);
-                        if (key_valid(d) && atomic_min_u64(&keys[ni], make_key(d, t, sign)))
+                        if (key_valid(d) && key_lower(&keys[ni], make_key(d, t, sign), sign))
                             __atomic_store_n(&changed[ni], 1, __ATOMIC_RELAXED);
                     }
         }
     }
     for (uint64_t i = 0; i < ncells; ++i) dist_arr[i] = key_value(keys[i], sign);
+    if (tri_out)
+        for (uint64_t i = 0; i < ncells; ++i) tri_out[i] = keys[i] == KEY_EMPTY ? UINT32_MAX : key_tri(keys[i], sign);
     free(keys);
     free(changed);
     free(fr_cell);
@@ -1210,6 +1248,14 @@ static void grid_raycast_sign(const ogrid* gp, const float* verts, const uint32_
 int orc_generate_grid_sdf(const float* verts, uint64_t nv, const uint32_t* idx, uint64_t ni, int topo,
                           const float* first_cell, const float* cell_size, const uint64_t* cell_count, int sign,
                           float* out, int n_heaps, int schedule, float* seed_out, orc_grid_stats* stats) {
+    return orc_generate_grid_sdf_ex(verts, nv, idx, ni, topo, first_cell, cell_size, cell_count, sign, out, NULL,
+                                    n_heaps, schedule, seed_out, stats);
+}
+
+int orc_generate_grid_sdf_ex(const float* verts, uint64_t nv, const uint32_t* idx, uint64_t ni, int topo,
+                             const float* first_cell, const float* cell_size, const uint64_t* cell_count, int sign,
+                             float* out, uint32_t* tri_out, int n_heaps, int schedule, float* seed_out,
+                             orc_grid_stats* stats) {
     if (n_heaps <= 0) n_heaps = 1;
     if (sign != ORC_SIGN_RAYCAST && sign != ORC_SIGN_NORMAL) return ORC_ERR_INVALID_ARGUMENT;
     orc_grid_stats lst;
@@ -1228,7 +1274,7 @@ int orc_generate_grid_sdf(const float* verts, uint64_t nv, const uint32_t* idx, 
     if (schedule == ORC_GRID_FRONTIER) {
         float* dist_arr = (float*)malloc(sizeof(float) * (ncells + 1));
         uint64_t evals = 0;
-        int64_t rounds = frontier_schedule(&g, verts, tris, nt, sign, dist_arr, seed_out, n_heaps, &evals);
+        int64_t rounds = frontier_schedule(&g, verts, tris, nt, sign, dist_arr, tri_out, seed_out, n_heaps, &evals);
         lst.rounds = rounds < 0 ? 0 : (uint64_t)rounds;
         lst.evals = evals;
         int st2 = rounds < 0 ? ORC_ERR_NAN_DISTANCE : ORC_OK;
@@ -1254,6 +1300,7 @@ int orc_generate_grid_sdf(const float* verts, uint64_t nv, const uint32_t* idx, 
         return ORC_ERR_INVALID_ARGUMENT;
     }
     for (uint64_t i = 0; i < ncells; ++i) {
+        if (tri_out) tri_out[i] = UINT32_MAX; /* no value */
         pre_d[i] = FLT_MAX; /* grid.rs:118-124 */
         pre_t[i] = 0;
         dist_arr[i] = FLT_MAX; /* grid.rs:137-143 */
@@ -1281,6 +1328,7 @@ int orc_generate_grid_sdf(const float* verts, uint64_t nv, const uint32_t* idx, 
         if (pre_d[i] < FLT_MAX) {
             uint64_t z = i % g.n[2], y = (i / g.n[2]) % g.n[1], x = i / (g.n[1] * g.n[2]);
             dist_arr[i] = pre_d[i];
+            if (tri_out) tri_out[i] = pre_t[i];
             ostate st0 = {pre_d[i], {(uint32_t)x, (uint32_t)y, (uint32_t)z}, pre_t[i]};
             states[k++] = st0;
         }
@@ -1310,10 +1358,10 @@ int orc_generate_grid_sdf(const float* verts, uint64_t nv, const uint32_t* idx, 
     if (threaded) {
 #pragma omp parallel for num_threads(n_heaps) schedule(static, 1)
         for (int hI = n_heaps - 1; hI >= 0; --hI)
-            jobs[hI].err = propagate(&g, verts, tris, jobs[hI].heap, dist_arr, sign, locks, &jobs[hI].pops);
+            jobs[hI].err = propagate(&g, verts, tris, jobs[hI].heap, dist_arr, tri_out, sign, locks, &jobs[hI].pops);
     } else {
         for (int hI = n_heaps - 1; hI >= 0; --hI)
-            jobs[hI].err = propagate(&g, verts, tris, jobs[hI].heap, dist_arr, sign, NULL, &jobs[hI].pops);
+            jobs[hI].err = propagate(&g, verts, tris, jobs[hI].heap, dist_arr, tri_out, sign, NULL, &jobs[hI].pops);
     }
     for (int hI = 0; hI < n_heaps; ++hI) {
         lst.pops += jobs[hI].pops;
@@ -1355,6 +1403,92 @@ void orc_nearest_bruteforce(const float* verts, const uint32_t* tris, uint64_t n
         d2_out[i] = bd;
         if (tri_out) tri_out[i] = bt;
     }
+}
+
+/* orc_nearest_bruteforce's result through the oracle's BVH (conservative
+ * pruning, so the same minimum FP32 d^2 and the same lowest triangle), for
+ * point sets where brute force would take too long. */
+void orc_nearest_bvh(const float* verts, const uint32_t* tris, uint64_t nt, const float* points, uint64_t np,
+                     float* d2_out, uint32_t* tri_out, int n_threads) {
+    if (n_threads <= 0) n_threads = 1;
+    obvh b;
+    bvh_build(&b, verts, tris, nt);
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)np; ++i) {
+        float bd;
+        uint32_t bt;
+        bvh_nearest(&b, verts, tris, ld(points + 3 * i), &bd, &bt);
+        d2_out[i] = bd;
+        if (tri_out) tri_out[i] = bt;
+    }
+    bvh_free(&b);
+}
+
+/* The state every schedule of the reference's propagation reaches
+ * (generate/grid.rs:495-558): when the last state a cell c was pushed with is
+ * popped, its triangle t(c) is offered to every neighbour n, and afterwards
+ * n's value only moves earlier in compare_distances' order, so at the end
+ *   (1) value(c) is the distance from c's centre to t(c), bit for bit
+ *       (|value| for Raycast, whose sign comes from the raycasts),
+ *   (2) no neighbour n has d(centre(n), t(c)) strictly before value(n) under
+ *       compare_distances (lib.rs:315-332; f32::MAX where n holds no value),
+ * for every cell c holding a value (tri_of[c] != UINT32_MAX). counts[0] and
+ * counts[1] receive the violations of (1) and (2), counts[2] the cells without
+ * a triangle and counts[3] those of them whose value is not f32::MAX. */
+int orc_grid_fixpoint_check(const float* verts, uint64_t nv, const uint32_t* idx, uint64_t ni, int topo,
+                            const float* first_cell, const float* cell_size, const uint64_t* cell_count, int sign,
+                            const float* values, const uint32_t* tri_of, int n_threads, uint64_t* counts) {
+    if (n_threads <= 0) n_threads = 1;
+    uint32_t* tris = NULL;
+    uint64_t nt = 0;
+    int s = decode_alloc(nv, idx, ni, topo, &tris, &nt);
+    if (s) return s;
+    ogrid g = {ld(first_cell), ld(cell_size), {cell_count[0], cell_count[1], cell_count[2]}};
+    const uint64_t ncells = g.n[0] * g.n[1] * g.n[2];
+    uint64_t bad_value = 0, not_fix = 0, no_tri = 0, no_tri_value = 0;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 1024) \
+    reduction(+ : bad_value, not_fix, no_tri, no_tri_value)
+    for (int64_t ii = 0; ii < (int64_t)ncells; ++ii) {
+        const uint64_t i = (uint64_t)ii;
+        const uint32_t t = tri_of[i];
+        const float v = sign == ORC_SIGN_RAYCAST ? fabsf(values[i]) : values[i];
+        if (t == UINT32_MAX) {
+            no_tri++;
+            if (!(v == FLT_MAX)) no_tri_value++;
+            continue;
+        }
+        if (t >= nt) {
+            bad_value++;
+            continue;
+        }
+        const uint64_t cz = i % g.n[2], cy = (i / g.n[2]) % g.n[1], cx = i / (g.n[1] * g.n[2]);
+        const uint64_t cell[3] = {cx, cy, cz};
+        const float dc = cell_distance(&g, verts, tris, t, cell, sign);
+        uint32_t b1, b2;
+        memcpy(&b1, &dc, 4);
+        memcpy(&b2, &v, 4);
+        if (b1 != b2) bad_value++;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int64_t x = (int64_t)cx + dx, y = (int64_t)cy + dy, z = (int64_t)cz + dz;
+                    if (x < 0 || y < 0 || z < 0 || x >= (int64_t)g.n[0] || y >= (int64_t)g.n[1] ||
+                        z >= (int64_t)g.n[2])
+                        continue;
+                    const uint64_t nc[3] = {(uint64_t)x, (uint64_t)y, (uint64_t)z};
+                    const uint64_t n = cell_idx(&g, nc[0], nc[1], nc[2]);
+                    const float dn = cell_distance(&g, verts, tris, t, nc, sign);
+                    const float vn = tri_of[n] == UINT32_MAX ? FLT_MAX
+                                     : (sign == ORC_SIGN_RAYCAST ? fabsf(values[n]) : values[n]);
+                    if (orc_compare_distances(dn, vn) == -1) not_fix++;
+                }
+    }
+    counts[0] = bad_value;
+    counts[1] = not_fix;
+    counts[2] = no_tri;
+    counts[3] = no_tri_value;
+    free(tris);
+    return ORC_OK;
 }
 
 typedef struct {

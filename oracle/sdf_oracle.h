@@ -91,9 +91,12 @@ typedef struct {
  * schedule ORC_GRID_THREADED: n_heaps OS threads run concurrently, as the
  * reference does (result depends on timing).
  * schedule ORC_GRID_FRONTIER: not one of the reference's schedules. The same
- * relaxation rule (a cell's triangle is tried on its 27 neighbours, kept on
- * strict improvement) run as rounds, each cell holding the minimum (distance
- * bits, triangle index) key. A cell is flagged when its key improves (seeded
+ * relaxation rule (a cell's triangle is tried on its 27 neighbours, kept when
+ * compare_distances calls it Less than the stored value) run as rounds, each
+ * cell holding the minimum (distance, triangle) under compare_distances' order
+ * with the lower triangle first among equal distances (that order is a strict
+ * total order, so the minimum does not depend on the order in which the
+ * candidates arrive). A cell is flagged when its value improves (seeded
  * cells start flagged). Round r takes the flagged cells whose distance is at
  * most T_r = m_r + band, where m_r is the smallest distance of any flagged cell
  * and band = ORC_GRID_BAND_CELLS * max|cell_size| (both in FP32), snapshots
@@ -118,6 +121,36 @@ int orc_generate_grid_sdf(const float* vertices, uint64_t n_vertices, const uint
                           const uint64_t* cell_count, int sign, float* out, int n_heaps, int schedule,
                           float* seed_out, orc_grid_stats* stats);
 
+/* orc_generate_grid_sdf plus, when tri_out is not NULL, the triangle each
+ * cell's value is the distance to (the topology-order index of the triangle
+ * the last improving state carried; UINT32_MAX for a cell with no value). */
+int orc_generate_grid_sdf_ex(const float* vertices, uint64_t n_vertices, const uint32_t* indices, uint64_t n_indices,
+                             int topology, const float* first_cell, const float* cell_size,
+                             const uint64_t* cell_count, int sign, float* out, uint32_t* tri_out, int n_heaps,
+                             int schedule, float* seed_out, orc_grid_stats* stats);
+
+/* Counts the violations of the state every schedule of the reference's
+ * propagation reaches (generate/grid.rs:495-558), given each cell's value and
+ * triangle: counts[0] cells whose value is not their triangle's distance bit
+ * for bit, counts[1] (cell, neighbour) pairs where the cell's triangle gives
+ * the neighbour a value compare_distances calls Less than the neighbour's,
+ * counts[2] cells without a triangle, counts[3] of those whose value is not
+ * f32::MAX. For Raycast, |value| is compared (the sign comes from the rays). */
+int orc_grid_fixpoint_check(const float* vertices, uint64_t n_vertices, const uint32_t* indices,
+                            uint64_t n_indices, int topology, const float* first_cell, const float* cell_size,
+                            const uint64_t* cell_count, int sign, const float* values, const uint32_t* tri_of,
+                            int n_threads, uint64_t* counts);
+
+/* Test-only fault injection into ORC_GRID_FRONTIER, so tests can show that
+ * the fixpoint check rejects a wrong schedule: ORC_FAULT_SKIP_DIRECTION never
+ * offers a cell's triangle to its +x neighbour, ORC_FAULT_DROP_DEFERRED drops
+ * the flagged cells beyond a round's band instead of keeping them,
+ * ORC_FAULT_PACKED_ORDER orders Normal-mode values by the packed key alone
+ * (|d|, then negative after positive) instead of compare_distances, which
+ * prefers a positive value over a negative one it approximately equals. */
+enum { ORC_FAULT_NONE = 0, ORC_FAULT_SKIP_DIRECTION = 1, ORC_FAULT_DROP_DEFERRED = 2, ORC_FAULT_PACKED_ORDER = 3 };
+void orc_set_fault(int kind);
+
 /* Vectors from each query to its closest point on the nearest triangle, plus the
  * 3-ray inside vote (generate/generic/rtree_bvh.rs:217-276). */
 int orc_query_vector_to_closest_point(const float* vertices, uint64_t n_vertices, const uint32_t* indices,
@@ -128,6 +161,9 @@ int orc_query_vector_to_closest_point(const float* vertices, uint64_t n_vertices
 /* exact FP32 nearest triangle (lowest index on ties) by brute force */
 void orc_nearest_bruteforce(const float* vertices, const uint32_t* tris, uint64_t n_tris, const float* points,
                             uint64_t n_points, float* d2_out, uint32_t* tri_out, int n_threads);
+/* the same through the oracle's BVH (identical results, for large point sets) */
+void orc_nearest_bvh(const float* vertices, const uint32_t* tris, uint64_t n_tris, const float* points,
+                     uint64_t n_points, float* d2_out, uint32_t* tri_out, int n_threads);
 /* FP64 brute-force unsigned distance (Ericson's closest point evaluated in double) */
 void orc_fp64_distance(const float* vertices, const uint32_t* tris, uint64_t n_tris, const float* points,
                        uint64_t n_points, double* out, int n_threads);
