@@ -49,13 +49,17 @@ void none_query(const TriRec* d_tris, uint64_t n_tris, const float* d_queries, u
 
 // Internal node: both children's boxes plus child links. A link >= 0 is an
 // internal node, a link < 0 is the leaf ~link (position in Morton order).
+// Node indices are Karras's (2012): node i covers the leaf positions
+// [first, last] with i at one end; its left child (link or leaf position) is
+// the split s, so the left child covers [first, s] and the right one
+// [s + 1, last], and a right child's index is its first position.
 struct __attribute__((aligned(16))) BvhNode {
     float lmin[3];
     int32_t left;
     float lmax[3];
     int32_t right;
     float rmin[3];
-    uint32_t pad0;
+    int32_t last;  // last leaf position the node covers
     float rmax[3];
     uint32_t pad1;
 };

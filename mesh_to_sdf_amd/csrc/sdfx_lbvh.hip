@@ -263,7 +263,7 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
 __global__ void __launch_bounds__(256) k_karras(const BuildSeg* __restrict__ segs, const uint32_t* __restrict__ skeys,
                                                 int32_t* __restrict__ g_left, int32_t* __restrict__ g_right,
                                                 int32_t* __restrict__ g_parent_int,
-                                                int32_t* __restrict__ g_parent_leaf) {
+                                                int32_t* __restrict__ g_parent_leaf, int32_t* __restrict__ g_last) {
     const BuildSeg& S = segs[blockIdx.y];
     const int64_t n = (int64_t)S.n;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -293,6 +293,7 @@ __global__ void __launch_bounds__(256) k_karras(const BuildSeg* __restrict__ seg
     int32_t rc = hi == gamma + 1 ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
     left[i] = lc;
     right[i] = rc;
+    g_last[S.node_off + i] = (int32_t)hi;
     if (lc < 0) parent_leaf[~lc] = (int32_t)i;
     else parent_int[lc] = (int32_t)i;
     if (rc < 0) parent_leaf[~rc] = (int32_t)i;
@@ -344,7 +345,8 @@ __global__ void __launch_bounds__(256) k_refit(const BuildSeg* __restrict__ segs
                                                const int32_t* __restrict__ g_parent_leaf,
                                                const int32_t* __restrict__ g_parent_int,
                                                const int32_t* __restrict__ g_left, const int32_t* __restrict__ g_right,
-                                               const Box* __restrict__ g_leaf_box, Box* g_node_box, uint32_t* g_visit) {
+                                               const Box* __restrict__ g_leaf_box, Box* g_node_box, uint32_t* g_visit,
+                                               const int32_t* __restrict__ g_last) {
     const BuildSeg& S = segs[blockIdx.y];
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (S.n < 2 || i >= S.n) return;
@@ -354,6 +356,7 @@ __global__ void __launch_bounds__(256) k_refit(const BuildSeg* __restrict__ segs
     const Box* leaf_box = g_leaf_box + S.tri_off;
     Box* node_box = g_node_box + S.node_off;
     uint32_t* visit = g_visit + S.node_off;
+    const int32_t* last = g_last + S.node_off;
     BvhNode* nodes = S.nodes;
     int32_t p = g_parent_leaf[S.tri_off + i];
     for (int level = 0; p >= 0 && level < 128; ++level) {  // depth <= 64 in a Karras tree
@@ -382,7 +385,7 @@ __global__ void __launch_bounds__(256) k_refit(const BuildSeg* __restrict__ segs
         }
         nd.left = l;
         nd.right = r;
-        nd.pad0 = 0;
+        nd.last = last[p];
         nd.pad1 = 0;
         nodes[p] = nd;
         p = parent_int[p];
@@ -436,10 +439,12 @@ void lbvh_build_batch(const std::vector<LbvhBuildJob>& jobs, Arena& A) {
     int32_t* parent_leaf = A.alloc<int32_t>(total);
     Box* node_box = A.alloc<Box>(total_nodes);
     uint32_t* visit = A.alloc<uint32_t>(total_nodes);
+    int32_t* last = A.alloc<int32_t>(total_nodes);
     fill_u32(visit, total_nodes, 0u, s);
-    SDFX_LAUNCH(k_karras, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, keys, left, right, parent_int, parent_leaf);
+    SDFX_LAUNCH(k_karras, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, keys, left, right, parent_int, parent_leaf,
+                last);
     SDFX_LAUNCH(k_refit, dim3(per_mesh, nseg), dim3(256), 0, s, d_segs, parent_leaf, parent_int, left, right,
-                leaf_box, node_box, visit);
+                leaf_box, node_box, visit, last);
     SDFX_HIP(hipGetLastError());
 }
 
@@ -449,14 +454,13 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
 
 // ------------------------------------------------------------------ traversal
 
-constexpr int STACK_DEPTH = 64;  // Karras depth <= 64 (32 code bits + 32 position bits)
-
-// Traversal stack of one lane: the newest STACK_RING entries live in LDS (a
-// ring in the lane's own column, so lanes never share words and need no
-// barrier), older ones spill to a private array (scratch memory) and return
-// when the ring drains. Entries are two u32 words. The traversal kernels run
-// 256-thread blocks; 8 entries x 8 bytes x 256 lanes = 16 KB of LDS per block
-// keeps 8 waves per SIMD resident.
+// Traversal stack of one lane: a ring of STACK_RING entries in LDS (the lane's
+// own column, so lanes never share words and need no barrier), two u32 words
+// per entry. A push onto a full ring overwrites the oldest entry and the
+// stack remembers that it dropped one; every traversal below recovers what
+// was dropped without other storage, so no traversal touches scratch memory.
+// The traversal kernels run 256-thread blocks: 8 entries x 8 bytes x 256
+// lanes = 16 KB of LDS per block keeps 8 waves per SIMD resident.
 constexpr int STACK_RING = 8;
 constexpr int STACK_LANES = 256;
 
@@ -465,21 +469,72 @@ __device__ __forceinline__ uint32_t* stack_lds() {
     return words;
 }
 
-struct LaneStack {
+struct RingStack {
     uint32_t* col;  // this lane's column: entry k at col[2 * k * STACK_LANES] / col[(2 * k + 1) * STACK_LANES]
-    uint32_t spill_a[STACK_DEPTH], spill_b[STACK_DEPTH];
+    int head = 0, count = 0;
+    bool dropped = false;  // an entry was overwritten since the flag was last cleared
+    __device__ RingStack() : col(stack_lds() + threadIdx.x) {}
+    __device__ __forceinline__ bool empty() const { return count == 0; }
+    // Returns true when the ring was full: the oldest entry was overwritten
+    // and its second word is in dropped_b.
+    __device__ __forceinline__ bool push(uint32_t a, uint32_t b, uint32_t& dropped_b) {
+        const int h = head & (STACK_RING - 1);  // with a full ring, the oldest entry's slot
+        const bool full = count == STACK_RING;
+        if (full) {
+            dropped_b = col[(2 * h + 1) * STACK_LANES];
+            dropped = true;
+            --count;
+        }
+        col[2 * h * STACK_LANES] = a;
+        col[(2 * h + 1) * STACK_LANES] = b;
+        ++head;
+        ++count;
+        return full;
+    }
+    __device__ __forceinline__ void pop(uint32_t& a, uint32_t& b) {  // requires !empty()
+        --head;
+        --count;
+        const int h = head & (STACK_RING - 1);
+        a = col[2 * h * STACK_LANES];
+        b = col[(2 * h + 1) * STACK_LANES];
+    }
+};
+
+// Stack of the first pass's nearest search: the newest STACK_RING entries in
+// the LDS ring, older ones in a private array (scratch memory) until the ring
+// drains. Unlike the other traversals this one keeps every entry: half of the
+// uniform queries of a noise sphere need more than eight pending boxes within
+// their final bound, and recovering dropped ones (a restart under the final
+// bound, keeping either the newest or the nearest eight) cost more than the
+// spill (measured on the 327,680-triangle sphere: 35-39 ms per 4.19M queries
+// against 24.5 ms; the spill writes about 360 bytes per query, 1% of HBM
+// bandwidth over the pass). The spill array is its own
+// object: a private array indexed at run time lives in scratch memory, and a
+// struct holding one would take the counters along, so every push and pop
+// would write them there; kept apart, the counters stay in registers.
+constexpr int SPILL_DEPTH = 64;  // Karras depth <= 64 (32 code bits + 32 position bits)
+
+struct StackSpill {
+    uint32_t a[SPILL_DEPTH], b[SPILL_DEPTH];
+};
+
+struct SpillStack {
+    uint32_t* col;  // this lane's ring column, as in RingStack
+    StackSpill& sp;
+    uint32_t& spills;  // statistics: entries moved to the spill array
     int head = 0, count = 0, nspill = 0;
-    __device__ LaneStack() : col(stack_lds() + threadIdx.x) {}
+    __device__ SpillStack(StackSpill& spill, uint32_t& n) : col(stack_lds() + threadIdx.x), sp(spill), spills(n) {}
     __device__ __forceinline__ bool empty() const { return count == 0 && nspill == 0; }
-    // false when the stack is full (STACK_DEPTH spilled entries): the entry is dropped
+    // false when the stack is full (SPILL_DEPTH spilled entries): the entry is dropped
     __device__ __forceinline__ bool push(uint32_t a, uint32_t b) {
         if (count == STACK_RING) {  // move the oldest ring entry to the spill array
-            if (nspill == STACK_DEPTH) return false;
+            if (nspill == SPILL_DEPTH) return false;
             const int old = head & (STACK_RING - 1);  // head - RING == head (mod RING)
-            spill_a[nspill] = col[2 * old * STACK_LANES];
-            spill_b[nspill] = col[(2 * old + 1) * STACK_LANES];
+            sp.a[nspill] = col[2 * old * STACK_LANES];
+            sp.b[nspill] = col[(2 * old + 1) * STACK_LANES];
             ++nspill;
             --count;
+            ++spills;
         }
         const int h = head & (STACK_RING - 1);
         col[2 * h * STACK_LANES] = a;
@@ -497,8 +552,8 @@ struct LaneStack {
             b = col[(2 * h + 1) * STACK_LANES];
         } else {
             --nspill;
-            a = spill_a[nspill];
-            b = spill_b[nspill];
+            a = sp.a[nspill];
+            b = sp.b[nspill];
         }
     }
 };
@@ -512,7 +567,14 @@ struct Nearest {
 // Traversal statistics gathered while kernel timing is on.
 struct TravStats {
     uint32_t nodes = 0, leaves = 0, vote_nodes = 0, vote_leaves = 0;
+    uint32_t spills = 0;     // nearest-search stack entries moved to scratch memory
+    uint32_t restarts = 0;   // restarts of exact traversals after the ring dropped an entry
 };
+
+// Visit caps: a tree visits each node once per pass, plus at most one descent
+// of at most 64 nodes per restart, and a restart follows a drop, which
+// follows a push of a visited node; more visits mean a malformed tree.
+__device__ __forceinline__ uint32_t exact_cap(uint32_t cap) { return cap >= (1u << 25) ? 0xFFFFFFFFu : cap * 65u; }
 
 __device__ __forceinline__ void visit_leaf_nearest(const LeafRec* __restrict__ leaves, int32_t leaf, float px,
                                                    float py, float pz, Nearest& best, TravStats& ts) {
@@ -526,41 +588,131 @@ __device__ __forceinline__ void visit_leaf_nearest(const LeafRec* __restrict__ l
     }
 }
 
-// Nearest search of one lane from `start` (a node, or ~leaf), closest child
-// first, folding into the lane's running best. Returns false when more than
-// `budget` nodes were visited with nodes left (the search is incomplete; best
-// holds what was found so far) if budget is below `cap`; more than `cap`
-// visits means a malformed tree (overflow).
+// Left-first depth-first traversal of the subtree at `start` (an internal
+// node), exact whatever the ring drops. With Karras's node numbering (see
+// BvhNode) the leaves are reached in increasing position, and `done` is a
+// position up to which every leaf has been visited or ruled out (start's first
+// position minus one on entry). When the ring drained after dropping an entry,
+// the traversal restarts at `start` and skips every left child that ends at or
+// before `done`: no leaf is visited twice and none is missed, and since each
+// drain finished at least one more subtree, `done` grows and the traversal
+// ends.
+//
+// P, the traversal's policy, decides what a child box needs and what a leaf
+// does: child(lo, hi, pay, child_pay) whether to enter a child box (pay is
+// what the path carries: the vote's live ray axes, or the box's distance for
+// pruning), keep(pay) whether a stacked entry is still wanted when popped (the
+// bound may have tightened since the push), leaf(position, pay), and
+// root_pay. Returns false when `budget` visits ran out below `cap` (the
+// search is incomplete); past exact_cap(cap) visits the tree is malformed.
+template <class P>
+__device__ bool dfs_exact(const BvhNode* __restrict__ nodes, int32_t start, int32_t done, P& pol,
+                          uint32_t& visits, uint32_t budget, uint32_t cap, bool& overflow, TravStats& ts) {
+    RingStack st;
+    int32_t node = start;
+    uint32_t pay = pol.root_pay;
+    const uint32_t limit = exact_cap(cap);
+    for (;;) {
+        // node: an internal node whose range extends beyond `done`
+        ++visits;
+        if (visits > budget && budget < cap) return false;
+        if (visits > limit) {
+            overflow = true;
+            return true;
+        }
+        pol.count_node();
+        const BvhNode nd = nodes[node];
+        const int32_t s = nd.left < 0 ? ~nd.left : nd.left;
+        uint32_t pl = 0, pr = 0;
+        bool gl = s > done && pol.child(nd.lmin, nd.lmax, pay, pl);
+        const bool gr = pol.child(nd.rmin, nd.rmax, pay, pr);
+        if (gl && nd.left < 0) {
+            pol.leaf(~nd.left, pl);
+            gl = false;
+        }
+        if (gl) {  // the right child, leaf or node, waits for the left subtree
+            uint32_t db;
+            if (gr) st.push((uint32_t)nd.right, pr, db);
+            node = nd.left;
+            pay = pl;
+            continue;
+        }
+        done = max(done, s);  // the left child is finished
+        if (gr) {
+            if (nd.right >= 0) {
+                node = nd.right;
+                pay = pr;
+                continue;
+            }
+            pol.leaf(~nd.right, pr);
+        }
+        done = max(done, nd.last);
+        // next: the newest stacked entry still wanted (a right child: its index
+        // is its first position), else a restart after a drop, else the end
+        bool found = false;
+        while (!st.empty()) {
+            uint32_t ea, eb;
+            st.pop(ea, eb);
+            const int32_t e = (int32_t)ea;
+            const int32_t first = e < 0 ? ~e : e;
+            done = max(done, first - 1);
+            if (!pol.keep(eb)) continue;  // the next entry or a restart moves `done` past it
+            if (e < 0) {
+                pol.leaf(first, eb);
+                done = first;
+                continue;
+            }
+            node = e;
+            pay = eb;
+            found = true;
+            break;
+        }
+        if (found) continue;
+        if (!st.dropped) return true;
+        st.dropped = false;
+        ts.restarts++;
+        node = start;
+        pay = pol.root_pay;
+    }
+}
+
+// Nearest search of one lane from the root, closest child first, folding
+// into the lane's running best. Returns false when more than `budget` nodes
+// were visited with nodes left (the search is incomplete; best holds what was
+// found so far) if budget is below `cap`; more than `cap` visits means a
+// malformed tree (overflow).
 //
 // Pruning keeps a box when box_d2 * PRUNE_SLACK <= bound, and bound is always
 // the d^2 of a triangle already seen, so a box is dropped only if every
 // triangle in it is strictly farther than one already seen. The answer is
 // therefore the minimum of (d^2, topology index) over all triangles whatever
-// the visiting order, start nodes or initial bound: k_lbvh_coop, which splits
-// one query's tree among the lanes of a wave, returns the same answer as one
-// lane.
-__device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t start,
+// the visiting order, start nodes, repeated visits or initial bound:
+// k_lbvh_coop, which splits one query's tree among the lanes of a wave,
+// returns the same answer as one lane.
+__device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                              float px, float py, float pz, uint32_t budget, uint32_t cap, Nearest& best,
                              bool& overflow, TravStats& ts) {
-    auto bound = [&]() { return best.d2; };
     auto leaf = [&](int32_t lf) { visit_leaf_nearest(leaves, lf, px, py, pz, best, ts); };
-    if (start < 0) {
-        leaf(~start);
+    if (root < 0) {
+        leaf(~root);
         return true;
     }
-    LaneStack st;
-    int32_t node = start;
-    for (uint32_t visits = 0;; ++visits) {
+    StackSpill spill;
+    SpillStack st(spill, ts.spills);
+    int32_t node = root;
+    uint32_t visits = 0;
+    for (;;) {
+        ++visits;
         if (visits > budget) {
             if (budget < cap) return false;
             overflow = true;  // a tree visits each node once; more means a malformed tree
-            break;
+            return true;
         }
         ts.nodes++;
         const BvhNode nd = nodes[node];
         float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
         float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
-        float b = bound();
+        float b = best.d2;
         bool gl = dl * PRUNE_SLACK <= b;
         bool gr = dr * PRUNE_SLACK <= b;
         // leaf children, left first; one call site, so lanes visiting the
@@ -572,11 +724,11 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* _
 #pragma unroll 1
         while (pending >= 0) {
             leaf(pending);
-            pending = then >= 0 && dr * PRUNE_SLACK <= bound() ? then : -1;  // the right leaf, pruned anew
+            pending = then >= 0 && dr * PRUNE_SLACK <= best.d2 ? then : -1;  // the right leaf, pruned anew
             then = -1;
         }
         if (gl && gr) {
-            bool left_first = dl <= dr;
+            const bool left_first = dl <= dr;
             if (!st.push((uint32_t)(left_first ? nd.right : nd.left), fbits(left_first ? dr : dl))) overflow = true;
             node = left_first ? nd.left : nd.right;
             continue;
@@ -593,7 +745,7 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* _
         while (!st.empty()) {
             uint32_t a, bb;
             st.pop(a, bb);
-            if (bitsf(bb) * PRUNE_SLACK <= bound()) {
+            if (bitsf(bb) * PRUNE_SLACK <= best.d2) {
                 node = (int32_t)a;
                 found = true;
                 break;
@@ -614,74 +766,50 @@ __device__ Nearest bvh_nearest(const BvhNode* __restrict__ nodes, const LeafRec*
 
 // Bvh(Normal) (generic/bvh.rs:117-129): min_dist starts at f32::MAX and takes
 // each candidate's signed distance when compare_distances(min_dist, d) is
-// Greater. Nodes are skipped only when their box is farther than any value
-// that could tie with (ulps 2 / eps 1e-6) or beat |min_dist|.
-__device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves,
-                                 int32_t root, float px, float py, float pz, uint32_t max_visits, bool& nan_seen,
-                                 bool& overflow) {
+// Greater. compare_distances is a strict weak order (a positive value comes
+// before the negative values it approximately equals, below a threshold that
+// grows with them), and candidates it calls Equal have the same sign and |d|,
+// i.e. the same value, so the fold's result does not depend on the order the
+// candidates come in, nor on a candidate coming twice: the traversal is the
+// left-first exact one. Nodes are skipped only when their box is farther than
+// any value that could tie with (ulps 2 / eps 1e-6) or beat |min_dist|.
+struct FoldPolicy {
+    const LeafRec* __restrict__ leaves;
+    float px, py, pz;
     float acc = FLT_MAX;
-    auto bound2 = [&](void) {
+    bool nan_seen = false;
+    static constexpr uint32_t root_pay = 0u;
+    __device__ float bound2() const {
         float b = fabsf(acc) * (1.0f + 1.0f / 1048576.0f) + 4e-6f;
         return b * b * (1.0f + 1.0f / 1048576.0f);
-    };
-    auto visit = [&](int32_t leaf) {
-        float d = tri_signed_dist(px, py, pz, expand_leaf(leaves[leaf]));
-        int c = compare_distances(acc, d);
+    }
+    __device__ void count_node() {}
+    __device__ bool child(const float* lo, const float* hi, uint32_t, uint32_t& cp) {
+        const float d2 = box_dist2(px, py, pz, lo, hi);
+        cp = fbits(d2);
+        return d2 <= bound2();
+    }
+    __device__ bool keep(uint32_t b) { return bitsf(b) <= bound2(); }
+    __device__ void leaf(int32_t pos, uint32_t) {
+        const float d = tri_signed_dist(px, py, pz, expand_leaf(leaves[pos]));
+        const int c = compare_distances(acc, d);
         nan_seen |= c == 2;
         if (c == 1) acc = d;
-    };
+    }
+};
+
+__device__ float bvh_normal_fold(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+                                 int32_t root, float px, float py, float pz, uint32_t max_visits, bool& nan_seen,
+                                 bool& overflow, TravStats& ts) {
+    FoldPolicy pol{leaves, px, py, pz};
     if (root < 0) {
-        visit(~root);
-        return acc;
+        pol.leaf(~root, 0u);
+    } else {
+        uint32_t visits = 0;
+        dfs_exact(nodes, root, -1, pol, visits, max_visits, max_visits, overflow, ts);
     }
-    LaneStack st;
-    int32_t node = root;
-    for (uint32_t visits = 0;; ++visits) {
-        if (visits > max_visits) {  // a tree visits each node once; more means a malformed tree
-            overflow = true;
-            break;
-        }
-        const BvhNode nd = nodes[node];
-        float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
-        float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
-        float b2 = bound2();
-        bool gl = dl <= b2, gr = dr <= b2;
-        if (gl && nd.left < 0) {
-            visit(~nd.left);
-            gl = false;
-            gr = dr <= bound2();
-        }
-        if (gr && nd.right < 0) {
-            visit(~nd.right);
-            gr = false;
-        }
-        if (gl && gr) {
-            bool left_first = dl <= dr;
-            if (!st.push((uint32_t)(left_first ? nd.right : nd.left), fbits(left_first ? dr : dl))) overflow = true;
-            node = left_first ? nd.left : nd.right;
-            continue;
-        }
-        if (gl) {
-            node = nd.left;
-            continue;
-        }
-        if (gr) {
-            node = nd.right;
-            continue;
-        }
-        bool found = false;
-        while (!st.empty()) {
-            uint32_t a, b;
-            st.pop(a, b);
-            if (bitsf(b) <= bound2()) {
-                node = (int32_t)a;
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
-    }
-    return acc;
+    nan_seen = pol.nan_seen;
+    return pol.acc;
 }
 
 __device__ __forceinline__ void ray_leaf(const LeafRec* __restrict__ leaves, int32_t leaf, uint32_t axes, float px,
@@ -693,58 +821,35 @@ __device__ __forceinline__ void ray_leaf(const LeafRec* __restrict__ leaves, int
     if ((axes & 4u) && ray_hit<2>(px, py, pz, t, nullptr)) cnt[2]++;
 }
 
-// The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198: the
-// three rays share one traversal; each stack entry carries its live axes.
+// Policy of dfs_exact for the 3-ray vote: the three rays share one traversal
+// and each path carries its live axes (a triangle must be counted exactly
+// once per ray, which the exact traversal guarantees).
+struct VotePolicy {
+    const LeafRec* __restrict__ leaves;
+    float px, py, pz;
+    TravStats& ts;
+    uint32_t cnt[3] = {0u, 0u, 0u};
+    static constexpr uint32_t root_pay = 7u;
+    __device__ void count_node() { ts.vote_nodes++; }
+    __device__ bool child(const float* lo, const float* hi, uint32_t axes, uint32_t& cp) {
+        cp = axes & ray_axes(lo, hi, px, py, pz);
+        return cp != 0u;
+    }
+    __device__ bool keep(uint32_t) { return true; }
+    __device__ void leaf(int32_t pos, uint32_t axes) { ray_leaf(leaves, pos, axes, px, py, pz, cnt, ts); }
+};
+
+// The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198.
 __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                                 float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
-    uint32_t cnt[3] = {0u, 0u, 0u};
+    VotePolicy pol{leaves, px, py, pz, ts};
     if (root < 0) {
-        ray_leaf(leaves, ~root, 7u, px, py, pz, cnt, ts);
+        pol.leaf(~root, 7u);
     } else {
-        LaneStack st;
-        int32_t node = root;
-        uint32_t axes = 7u;
-        for (uint32_t visits = 0;; ++visits) {
-            if (visits > max_visits) {
-                overflow = true;
-                break;
-            }
-            ts.vote_nodes++;
-            const BvhNode nd = nodes[node];
-            uint32_t al = axes & ray_axes(nd.lmin, nd.lmax, px, py, pz);
-            uint32_t ar = axes & ray_axes(nd.rmin, nd.rmax, px, py, pz);
-            if (al && nd.left < 0) {
-                ray_leaf(leaves, ~nd.left, al, px, py, pz, cnt, ts);
-                al = 0;
-            }
-            if (ar && nd.right < 0) {
-                ray_leaf(leaves, ~nd.right, ar, px, py, pz, cnt, ts);
-                ar = 0;
-            }
-            if (al && ar) {
-                if (!st.push((uint32_t)nd.right, ar)) overflow = true;
-                node = nd.left;
-                axes = al;
-                continue;
-            }
-            if (al) {
-                node = nd.left;
-                axes = al;
-                continue;
-            }
-            if (ar) {
-                node = nd.right;
-                axes = ar;
-                continue;
-            }
-            if (st.empty()) break;
-            uint32_t a, b;
-            st.pop(a, b);
-            node = (int32_t)a;
-            axes = b;
-        }
+        uint32_t visits = 0;
+        dfs_exact(nodes, root, -1, pol, visits, max_visits, max_visits, overflow, ts);
     }
-    int insides = (int)(cnt[0] & 1u) + (int)(cnt[1] & 1u) + (int)(cnt[2] & 1u);
+    const int insides = (int)(pol.cnt[0] & 1u) + (int)(pol.cnt[1] & 1u) + (int)(pol.cnt[2] & 1u);
     return insides > 1;
 }
 
@@ -797,23 +902,23 @@ template <int MODE>
 __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves,
                                             int32_t root, uint32_t max_visits, const float* __restrict__ q,
                                             uint64_t i, float* __restrict__ out, DevFlags* flags, TravStats& ts,
-                                            uint32_t budget = 0xFFFFFFFFu, DeferRec* defer = nullptr) {
+                                            uint32_t budget, bool can_defer, DeferRec& defer) {
     const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
     bool overflow = false, deferred = false;
     if (MODE == QM_NORMAL_FOLD) {
         bool nan_seen = false;
-        out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow);
+        out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow, ts);
         if (nan_seen) atomicOr(&flags->nan_distance, 1u);
     } else {
         Nearest nb{INFINITY, 0xFFFFFFFFu, -1};
-        const bool done = nearest_from(nodes, leaves, root, px, py, pz, defer ? min(budget, max_visits) : max_visits,
+        const bool done = nearest_from(nodes, leaves, root, px, py, pz, can_defer ? min(budget, max_visits) : max_visits,
                                        max_visits, nb, overflow, ts);
         const bool inside = MODE != QM_NEAREST_SIGNED &&
                             bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
         if (done) {
             out[i] = nearest_answer<MODE>(nb, inside, px, py, pz, leaves);
         } else {
-            *defer = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u, 0u, 0u};
+            defer = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u, 0u, 0u};
             deferred = true;
         }
     }
@@ -822,7 +927,8 @@ __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, c
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ nodes,
+// 8 waves per SIMD (at most 64 VGPRs; 65 and 7 waves otherwise).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_lbvh_query(const BvhNode* __restrict__ nodes,
                                                     const LeafRec* __restrict__ leaves, int32_t root,
                                                     uint32_t max_visits, const float* __restrict__ q, uint64_t nq,
                                                     const uint32_t* __restrict__ order, float* __restrict__ out,
@@ -836,8 +942,8 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
     DeferRec rec;
     if (active) {
         if (order) i = order[i];  // lanes of a wave take Morton-adjacent queries
-        dfr = query_point<MODE>(nodes, leaves, root, max_visits, q, i, out, flags, ts, budget,
-                                deferred ? &rec : nullptr);
+        dfr = query_point<MODE>(nodes, leaves, root, max_visits, q, i, out, flags, ts, budget, deferred != nullptr,
+                                rec);
     }
     if (deferred) {  // kernel argument: uniform; one atomic per wave appends its deferred queries
         const uint64_t m = __ballot(dfr);
@@ -865,6 +971,8 @@ __global__ void __launch_bounds__(256) k_lbvh_query(const BvhNode* __restrict__ 
         }
         block_count(stats + 4 * nb, (threadIdx.x & 63u) == 0 ? 64u * mx : 0u);
         block_count(stats + 5 * nb, (threadIdx.x & 63u) == 0 ? 64u * mv : 0u);
+        block_count(stats + 6 * nb, ts.spills);
+        block_count(stats + 7 * nb, ts.restarts);
     }
 }
 
@@ -885,13 +993,18 @@ struct PointReq {
 constexpr uint32_t COOP_POOL = 128;
 
 // The deferred nearest searches, one wave per query, wave-synchronous: each
-// step every busy lane visits one node (closest child first, the farther
-// child kept on its own stack), idle lanes take farther children that busy
+// step every busy lane visits one node, idle lanes take subtrees that busy
 // lanes donate to a pool in LDS while the pool holds fewer than 64 entries,
-// and nodes are pruned against the wave's minimum d^2. The answer is the
-// wave's minimum of (d^2, topology index), starting from the best triangle
-// the first pass found (nearest_from explains why the order does not matter).
-// Persistent waves take queries from *next until all *n_deferred are done.
+// and nodes are pruned against the wave's minimum d^2. A lane walks its
+// subtree left child first, keeping the right children it does not donate on
+// its ring; after the ring dropped an entry, the lane restarts at its subtree
+// once the ring drains, skipping the left children that end at or before the
+// last position it finished (dfs_exact explains the numbering; donated right
+// children are all right children, so a subtree's index is its first
+// position). The answer is the wave's minimum of (d^2, topology index),
+// starting from the best triangle the first pass found (nearest_from explains
+// why the order, and visiting a subtree twice, do not matter). Persistent
+// waves take queries from *next until all *n_deferred are done.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ reqs, PointReq single,
                                                    const DeferRec* __restrict__ recs, const uint32_t* n_deferred,
@@ -911,12 +1024,14 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
         const PointReq R = reqs ? reqs[r.req] : single;
         const BvhNode* __restrict__ nodes = R.nodes;
         const LeafRec* __restrict__ leaves = R.leaves;
-        const uint32_t max_visits = R.max_visits;
+        const uint32_t limit = exact_cap(R.max_visits);
         const float px = R.q[3 * r.q], py = R.q[3 * r.q + 1], pz = R.q[3 * r.q + 2];
         Nearest best = lane == 0 ? Nearest{bitsf(r.d2), r.index, r.leaf} : Nearest{INFINITY, 0xFFFFFFFFu, -1};
-        LaneStack st;
-        int32_t cur = -1;  // the lane's node, -1 when idle (its stack is then empty)
-        uint32_t pn = 1;   // pool entries: the same value in every lane
+        RingStack st;
+        int32_t cur = -1;   // the lane's node, -1 when idle (its ring is then empty)
+        int32_t sub = -1;   // the lane's subtree (restart point)
+        int32_t done = -1;  // leaf position up to which the lane's subtree is finished
+        uint32_t pn = 1;    // pool entries: the same value in every lane
         if (lane == 0) {
             pool_node[w][0] = (uint32_t)R.root;
             pool_d2[w][0] = 0u;
@@ -931,7 +1046,12 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
                 const uint32_t rk = (uint32_t)__popcll(idle & below);
                 if (rk < pn) {
                     const uint32_t slot = pn - 1u - rk;
-                    if (bitsf(pool_d2[w][slot]) * PRUNE_SLACK <= bound) cur = (int32_t)pool_node[w][slot];
+                    if (bitsf(pool_d2[w][slot]) * PRUNE_SLACK <= bound) {
+                        cur = sub = (int32_t)pool_node[w][slot];
+                        done = sub - 1;
+                        st.count = 0;  // empty already unless a malformed tree cut the last subtree short
+                        st.dropped = false;
+                    }
                 }
             }
             pn -= min(pn, (uint32_t)__popcll(idle));
@@ -939,16 +1059,18 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
             bool has_far = false;
             uint32_t far_node = 0, far_d2 = 0;
             if (cur >= 0) {
-                if (++visits > max_visits) {  // a tree visits each node once; more means a malformed tree
+                if (++visits > limit) {  // see exact_cap: a malformed tree
                     overflow = true;
                     cur = -1;
                 } else {
                     ts.nodes++;
+                    const float bnd = fminf(bound, best.d2);
                     const BvhNode nd = nodes[cur];
+                    const int32_t s = nd.left < 0 ? ~nd.left : nd.left;
                     const float dl = box_dist2(px, py, pz, nd.lmin, nd.lmax);
                     const float dr = box_dist2(px, py, pz, nd.rmin, nd.rmax);
-                    bool gl = dl * PRUNE_SLACK <= fminf(bound, best.d2);
-                    bool gr = dr * PRUNE_SLACK <= fminf(bound, best.d2);
+                    bool gl = s > done && dl * PRUNE_SLACK <= bnd;
+                    bool gr = dr * PRUNE_SLACK <= bnd;
                     const int32_t l0 = gl && nd.left < 0 ? ~nd.left : -1, l1 = gr && nd.right < 0 ? ~nd.right : -1;
                     if (l0 >= 0) gl = false;
                     if (l1 >= 0) gr = false;
@@ -959,39 +1081,47 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
                         pending = then >= 0 && dr * PRUNE_SLACK <= fminf(bound, best.d2) ? then : -1;
                         then = -1;
                     }
-                    if (gl && gr) {
-                        const bool left_first = dl <= dr;
-                        cur = left_first ? nd.left : nd.right;
-                        far_node = (uint32_t)(left_first ? nd.right : nd.left);
-                        far_d2 = fbits(left_first ? dr : dl);
-                        has_far = true;
-                    } else if (gl) {
+                    if (gl) {
                         cur = nd.left;
+                        if (gr) {
+                            far_node = (uint32_t)nd.right;
+                            far_d2 = fbits(dr);
+                            has_far = true;
+                        }
                     } else if (gr) {
+                        done = max(done, s);
                         cur = nd.right;
                     } else {
+                        done = max(done, nd.last);
                         cur = -1;
                         while (!st.empty()) {
                             uint32_t a, bb;
                             st.pop(a, bb);
+                            done = max(done, (int32_t)a - 1);  // a right child: its index is its first position
                             if (bitsf(bb) * PRUNE_SLACK <= fminf(bound, best.d2)) {
                                 cur = (int32_t)a;
                                 break;
                             }
                         }
+                        if (cur < 0 && st.dropped) {  // restart the subtree, skipping what is finished
+                            st.dropped = false;
+                            ts.restarts++;
+                            cur = sub;
+                        }
                     }
                 }
             }
-            // farther children go to the pool while it runs low (at most
-            // 63 + 64 < COOP_POOL entries), else to the lane's own stack
+            // right children go to the pool while it runs low (at most
+            // 63 + 64 < COOP_POOL entries), else to the lane's own ring
             const uint64_t donors = __ballot(has_far);
             if (has_far) {
                 if (pn < 64u) {
                     const uint32_t slot = pn + (uint32_t)__popcll(donors & below);
                     pool_node[w][slot] = far_node;
                     pool_d2[w][slot] = far_d2;
-                } else if (!st.push(far_node, far_d2)) {
-                    overflow = true;
+                } else {
+                    uint32_t db;
+                    st.push(far_node, far_d2, db);
                 }
             }
             if (pn < 64u) pn += (uint32_t)__popcll(donors);
@@ -1015,6 +1145,7 @@ __global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ 
     if (stats) {
         block_count(stats, ts.nodes);
         block_count(stats + gridDim.x, ts.leaves);
+        block_count(stats + 2 * gridDim.x, ts.restarts);
     }
 }
 
@@ -1119,8 +1250,8 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     unsigned long long* stats = nullptr;
     const unsigned coop_blocks = (unsigned)std::min<uint64_t>(2048, (nq + 3) / 4);
     if (profile_enabled() && mode != QM_NORMAL_FOLD) {
-        stats = A.alloc<unsigned long long>(6ull * blocks + 2ull * coop_blocks);
-        fill_u32(reinterpret_cast<uint32_t*>(stats), 12ull * blocks + 4ull * coop_blocks, 0u, s);
+        stats = A.alloc<unsigned long long>(8ull * blocks + 3ull * coop_blocks);
+        fill_u32(reinterpret_cast<uint32_t*>(stats), 16ull * blocks + 6ull * coop_blocks, 0u, s);
     }
     const uint32_t b32 = split ? (uint32_t)budget : 0xFFFFFFFFu;
     switch (mode) {
@@ -1139,7 +1270,7 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         case QM_CLOSEST_VECTOR:  // vectors have their own entry point, lbvh_closest_vectors
             throw Error(SDFX_ERR_INVALID_ARGUMENT, "closest-point vectors are not an SDF query mode");
     }
-    unsigned long long* cstats = stats ? stats + 6ull * blocks : nullptr;
+    unsigned long long* cstats = stats ? stats + 8ull * blocks : nullptr;
     if (split) {
         const PointReq single{bvh.nodes.get(), bvh.leaves.get(), d_q, d_out, 0, bvh.root, visit_cap(bvh), 0u, 0u};
         launch_coop(mode, nullptr, single, words, recs, coop_blocks, d_flags, cstats, s);
@@ -1152,10 +1283,13 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         profile_count("lbvh.vote_leaves", sum_block_counters(stats + 3ull * blocks, blocks, s));
         profile_count("lbvh.nearest_wave_slots", sum_block_counters(stats + 4ull * blocks, blocks, s));
         profile_count("lbvh.vote_wave_slots", sum_block_counters(stats + 5ull * blocks, blocks, s));
+        profile_count("lbvh.nearest_spills", sum_block_counters(stats + 6ull * blocks, blocks, s));
+        profile_count("lbvh.restarts", sum_block_counters(stats + 7ull * blocks, blocks, s));
         profile_count("lbvh.queries", nq);
         if (split) {
             profile_count("lbvh.deferred_nodes", sum_block_counters(cstats, coop_blocks, s));
             profile_count("lbvh.deferred_leaves", sum_block_counters(cstats + coop_blocks, coop_blocks, s));
+            profile_count("lbvh.deferred_restarts", sum_block_counters(cstats + 2ull * coop_blocks, coop_blocks, s));
             uint32_t nd = 0;
             SDFX_HIP(hipMemcpyAsync(&nd, words, sizeof(nd), hipMemcpyDeviceToHost, s));
             SDFX_HIP(hipStreamSynchronize(s));
@@ -1205,7 +1339,9 @@ __global__ void __launch_bounds__(256) k_point_morton_batch(const PointReq* __re
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
+// 7 waves per SIMD (at most 72 VGPRs): left to itself the compiler takes 77
+// and 6 waves, which measured slower on the served workload.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
                                                           uint64_t total, const uint32_t* __restrict__ order,
                                                           DevFlags* flags, DeferRec* __restrict__ deferred,
                                                           uint32_t* n_deferred) {
@@ -1218,7 +1354,7 @@ __global__ void __launch_bounds__(256) k_lbvh_query_batch(const PointReq* __rest
         TravStats ts;
         auto run = [&](const PointReq& R) {
             dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag,
-                                    ts, R.budget, deferred ? &rec : nullptr);
+                                    ts, R.budget, deferred != nullptr, rec);
         };
         // queries lie request by request: a wave nearly always holds one, whose
         // fields (tree pointers among them) then sit in scalar registers
