@@ -249,3 +249,31 @@ def config4(n_meshes=256, seed=SEED_BASE + 4, t_min=1_280, t_max=327_680, edge_m
         first, size, count = grid_for(v, counts)
         out.append((v, f, first, size, count))
     return out
+
+
+def serve_requests(meshes, n_requests=256, seed=SEED_BASE + 6, point_min=1_000, point_max=1_000_000,
+                   edge_min=32, edge_max=128):
+    """The served workload's request stream over `meshes` (a list of
+    (vertices, faces, ...) as config4 returns): requests alternate point
+    queries and grids, each on a mesh drawn uniformly. A point request holds n
+    points, n log-uniform in [point_min, point_max], half near the surface
+    (normal offsets N(0, 0.01 * bbox diagonal)) and half uniform in the AABB
+    expanded by 10%; a grid request has e^3 cells, e log-uniform in
+    [edge_min, edge_max], over the AABB expanded by 10%. Returns a list of
+    ("points", mesh, points) and ("grid", mesh, (first_cell, cell_size,
+    cell_count)); the points are generated here, so large streams take memory
+    (about 12 bytes per point)."""
+    g = _rng(seed)
+    out = []
+    for k in range(n_requests):
+        mi = int(g.integers(len(meshes)))
+        v, f = meshes[mi][0], meshes[mi][1]
+        if k % 2 == 0:
+            n = int(round(math.exp(g.uniform(math.log(point_min), math.log(point_max)))))
+            q = np.concatenate([surface_queries(v, f, n // 2, 0.01 * bbox_diagonal(v), seed + 10 + k),
+                                uniform_queries(v, n - n // 2, seed + 20 + k)])
+            out.append(("points", mi, q))
+        else:
+            e = int(round(math.exp(g.uniform(math.log(edge_min), math.log(edge_max)))))
+            out.append(("grid", mi, grid_for(v, (e, e, e))))
+    return out

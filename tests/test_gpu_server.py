@@ -1,7 +1,8 @@
 """The served form (sdfx_server_*): each answer is bit-identical to the same
-request answered alone through the plain entry points, requests queued while
-the device is busy are served in shared batches, the HBM budget gates meshes
-and requests, and a failing request leaves the others untouched."""
+request answered alone through the plain entry points and to the oracle,
+requests queued while the device is busy are served in shared batches, the
+HBM budget gates meshes and requests, and a failing request leaves the others
+untouched."""
 import numpy as np
 import pytest
 
@@ -24,10 +25,24 @@ def _server(budget=4 << 30, workers=2):
     return SdfServer(budget, workers)
 
 
+def _oracle_answer(oracle, v, f, kind, a, b):
+    """The oracle's answer to a served request: generate_sdf for points (method
+    b), the frontier schedule for grids (sign b), which the plain grid path
+    matches bit for bit."""
+    if kind == "grid":
+        s, ref, _ = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, a.first_cell, a.cell_size, a.cell_count, int(b),
+                                             16, oracle.GRID_FRONTIER)
+    else:
+        s, ref = oracle.generate_sdf(v, f, oracle.TOPO_LIST, a, b.kind, b.sign)
+    assert s in (0, 1)
+    return ref
+
+
 @pytest.mark.parametrize("visit_budget", [1024, 16])
-def test_mixed_requests_equal_alone(sdf, visit_budget):
+def test_mixed_requests_equal_alone(sdf, oracle, visit_budget):
     """Every method and both grid signs; with a 16-visit budget nearly every
-    nearest search of the batched point pass finishes in the per-wave pass."""
+    nearest search of the batched point pass finishes in the per-wave pass.
+    Each answer equals the plain entry point's and the oracle's, bit for bit."""
     meshes = _meshes()
     g = np.random.default_rng(11)
     methods = list(sdf.ALL_SDF_ACCELERATION_METHODS)
@@ -57,8 +72,64 @@ def test_mixed_requests_equal_alone(sdf, visit_budget):
             else:
                 alone = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), a, b)
             assert np.array_equal(bits(got.cpu().numpy()), bits(alone)), (kind, mi, b)
+            ref = _oracle_answer(oracle, v, f, kind, a, b)
+            assert np.array_equal(bits(got.cpu().numpy()), bits(ref)), ("oracle", kind, mi, b)
         st = srv.stats()
         assert st["requests_done"] == len(jobs) and st["queued"] == 0 and st["inflight_bytes"] == 0
+
+
+def test_served_workload_alone_and_oracle(sdf, oracle):
+    """A reduced served workload: 64 resident meshes drawn as in the batched
+    workload (1,280-327,680 triangles, one batched admission), 32 requests
+    drawn as the served stream draws them (points 1k-1M with Bvh(Raycast),
+    grids 32^3-128^3 with Raycast) plus one 1M-point request and one 128^3 grid
+    on the largest mesh, submitted in random order at Poisson arrivals (1,000
+    requests/s) so that they join batches already running. Each answer equals
+    the same request answered alone by the plain entry points, bit for bit;
+    the requests of at most 100,000 points or 64^3 cells also equal the
+    oracle's answer."""
+    import time
+    meshes = S.config4(64)
+    reqs = S.serve_requests(meshes, 32, seed=S.SEED_BASE + 7)
+    big = max(range(len(meshes)), key=lambda i: len(meshes[i][1]))
+    vb, fb = meshes[big][0], meshes[big][1]
+    reqs.append(("points", big, np.concatenate([S.surface_queries(vb, fb, 500_000, 0.01 * S.bbox_diagonal(vb), 31),
+                                                S.uniform_queries(vb, 500_000, 32)])))
+    reqs.append(("grid", big, S.grid_for(vb, (128, 128, 128))))
+    method = sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast)
+    g = np.random.default_rng(6)
+    with _server(budget=32 << 30) as srv:
+        ids = srv.add_meshes([(v, sdf.Topology.TriangleList(f)) for v, f, *_ in meshes])
+        jobs = []
+        for k in g.permutation(len(reqs)):
+            kind, mi, a = reqs[k]
+            time.sleep(float(g.exponential(1e-3)))
+            if kind == "points":
+                jobs.append((kind, mi, a, srv.submit_points(ids[mi], a, method)))
+            else:
+                grid = sdf.Grid(*a)
+                jobs.append((kind, mi, grid, srv.submit_grid(ids[mi], grid, sdf.SignMethod.Raycast)))
+        checked = 0
+        for kind, mi, a, tk in jobs:
+            got, _ = srv.wait(tk)
+            got = got.cpu().numpy()
+            v, f = meshes[mi][0], meshes[mi][1]
+            topo = sdf.Topology.TriangleList(f)
+            if kind == "points":
+                alone = sdf.generate_sdf(v, topo, a, method)
+                small = len(a) <= 100_000
+            else:
+                alone = sdf.generate_grid_sdf(v, topo, a, sdf.SignMethod.Raycast)
+                small = a.get_total_cell_count() <= 64 ** 3
+            assert np.array_equal(bits(got), bits(alone)), (kind, mi)
+            if small:
+                ref = _oracle_answer(oracle, v, f, kind, a, method if kind == "points" else sdf.SignMethod.Raycast)
+                assert np.array_equal(bits(got), bits(ref)), ("oracle", kind, mi)
+                checked += 1
+        st = srv.stats()
+    print(f"served workload: {len(jobs)} requests in {st['batches']} batches (largest {st['max_batch_requests']}), "
+          f"{checked} checked against the oracle")
+    assert st["requests_done"] == len(jobs) and checked >= 8
 
 
 def test_queued_requests_share_batches(sdf):
