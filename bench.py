@@ -1,28 +1,33 @@
 """SDF samples/s of the MI355X path on one GPU.
 
-Default workload (the metric's grid configuration): generate_grid_sdf with
-SignMethod::Raycast on a jittered torus (R=1, r=0.35, 256x128 segments, 65,536
-triangles) over a 128^3 grid built from its AABB expanded by 10%; one step =
-one full generate_grid_sdf call (topology decode, seeding, propagation to the
-fixpoint, raycast sign, output write) on vertex/index buffers already resident
-in HBM. One SDF sample = one output cell.
-
-    python bench.py [--steps K] [--warmup W] [--workload grid|rtree_bvh|bvh|grid_batch]
-
-grid_batch: one step = one sdfx_generate_grid_sdf_batch_device call over 256
-requests; the same requests are also run as a plain loop of
-sdfx_generate_grid_sdf_device calls and both forms are reported (samples/s,
-p50/p99 latency per request).
-
-serve: one step = the 256 requests submitted at once to a server
-(sdfx_server_*) holding the 64 meshes, until the last is answered (saturated
-throughput). Also reported: Poisson arrivals at each rate in SERVE_LAMBDAS
+Default workload (the metric's served form): a server (sdfx_server_*) holding
+64 resident meshes drawn as in the batched workload (jittered icospheres and
+tori of 1,280-327,680 triangles, admitted with one batched LBVH build) answers
+256 requests drawn as synthetic.serve_requests draws them: half point queries
+(1k-1M points, log-uniform, half near-surface) with Bvh(Raycast), half grids
+(32^3-128^3, log-uniform edge) with Raycast. One step = the 256 requests
+submitted at once, until the last is answered (saturated throughput); one
+SDF sample = one output float (a point's distance or a grid cell). Also
+reported under `forms`: Poisson arrivals at each rate in SERVE_LAMBDAS
 (samples/s, p50/p99 latency from submission to completion), and the plain
 drop-in (sdfx_generate_sdf_device / sdfx_generate_grid_sdf_device, which
 rebuild their acceleration per call) running the same requests one by one.
+Inputs are resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--workload serve|grid|grid_batch|rtree_bvh|bvh|rtree|none|mesh_check]
+
+grid: one step = one generate_grid_sdf (Raycast) on a jittered torus (R=1,
+r=0.35, 256x128 segments, 65,536 triangles) over a 128^3 grid from its AABB
+expanded by 10%. grid_batch: one step = one sdfx_generate_grid_sdf_batch_device
+call over 256 requests, reported beside the plain loop of single calls.
+rtree_bvh / bvh / rtree / none: one generate_sdf call per step.
 
 Prints one JSON line. The CPU baseline is the oracle's restatement of the
-reference's threaded algorithm, timed on this host's cores.
+reference's threaded algorithm, timed on this host's cores (the CPU share the
+machine gives one GPU) on a bounded sample; with it comes an accuracy block:
+sign mismatches against the oracle with the samples within 1e-5 x bbox
+diagonal of the surface excluded (and counted), and the error against an FP64
+brute-force distance for the GPU and the CPU alike.
 """
 import argparse
 import ctypes
@@ -56,10 +61,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="grid", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="serve", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the HIP-event kernel-timing pass (for runs under an external profiler)")
+    ap.add_argument("--burst-only", action="store_true",
+                    help="serve: time the burst steps only (no Poisson sweep or plain run), for profiler passes "
+                         "that must see only the timed region's launches")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the reference for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -272,7 +280,7 @@ def main():
                                 "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                                 "latency": "per request (one sdfx_generate_grid_sdf_device call)"},
                  "batched_over_plain": value / (samples / loop_s)}
-    if args.workload == "serve":
+    if args.workload == "serve" and not args.burst_only:
         forms = serve.forms(value, ms, step_ms)
 
     # Kernel timing: the same K steps again with HIP events around every launch
@@ -302,17 +310,24 @@ def main():
     # vertex and per index triple) is read once per call. One call runs
     # `launches_per_step` launches of the dominant kernel, which are charged
     # the call's bytes in equal parts.
-    per_sample = (0.0 if args.workload == "mesh_check" else 4.0 if args.workload.startswith("grid") else
-                  (4.0 * serve.grid_samples + 16.0 * serve.point_samples) / samples if args.workload == "serve"
-                  else 16.0)
-    step_bytes = per_sample * samples + mesh_bytes
+    per_sample = 0.0 if args.workload == "mesh_check" else 4.0 if args.workload.startswith("grid") else 16.0
+    work_samples, work_mesh = samples, mesh_bytes
+    if args.workload == "serve":
+        # each kernel is charged the requests it serves: the LBVH query kernels
+        # the point requests (16 B per point plus their meshes), the grid
+        # kernels the grid requests (4 B per cell plus their meshes)
+        if "lbvh" in dom_name:
+            per_sample, work_samples, work_mesh = 16.0, serve.point_samples, serve.point_mesh_bytes
+        else:
+            per_sample, work_samples, work_mesh = 4.0, serve.grid_samples, serve.grid_mesh_bytes
+    step_bytes = per_sample * work_samples + work_mesh
     launches_per_step = dom_launches / args.steps
     avg_launch_s = dom_ms / dom_launches / 1e3
     achieved = step_bytes / launches_per_step / avg_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
                 "traffic": measured_traffic(args.workload, dom_name), "kernel": dom_name, "avg_launch_ms": dom_ms / dom_launches,
-                "work_per_launch": f"{step_bytes / launches_per_step:.0f} B ({per_sample:.0f} B x {samples} samples "
-                                   f"+ {mesh_bytes} B of mesh, over {launches_per_step:g} launches per call)"}
+                "work_per_launch": f"{step_bytes / launches_per_step:.0f} B ({per_sample:.0f} B x {work_samples} "
+                                   f"samples + {work_mesh} B of mesh, over {launches_per_step:g} launches per call)"}
     valu = None
     # FP32 VALU ceiling of this code: without FMA contraction (the reference's
     # rounding) and without packed FP32, one lane retires one FP32 operation
@@ -340,6 +355,8 @@ def main():
                     "frac_of_unfused_ceiling": tflops / valu_ceiling, "evals_per_launch": evals / dom_launches}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, v, f, locals())
+    if cpu is not None and "host" not in cpu:
+        cpu["host"] = host_cpus()[1]
 
     checking = args.workload == "mesh_check"
     result = {
@@ -413,27 +430,26 @@ class ServeWorkload:
         _lib.check(L.sdfx_server_add_meshes(h, descs, n_meshes))
         self.pool_build_ms = (time.perf_counter() - t0) * 1e3
         self.ids = [descs[i].id for i in range(n_meshes)]
-        g = np.random.Generator(np.random.PCG64(seed + 1))
         self.reqs = []
         self.point_samples = self.grid_samples = 0
-        for k in range(n_requests):
-            mi = int(g.integers(n_meshes))
+        self.point_mesh_bytes = 0  # mesh bytes read by the point requests (12 B per vertex and per index triple)
+        self.grid_mesh_bytes = 0
+        for kind, mi, a in S.serve_requests(self.meshes, n_requests, seed=seed + 1):
             mv, mf = self.meshes[mi][0], self.meshes[mi][1]
-            if k % 2 == 0:
-                n = int(round(math.exp(g.uniform(math.log(1e3), math.log(1e6)))))
-                q = np.concatenate([S.surface_queries(mv, mf, n // 2, 0.01 * S.bbox_diagonal(mv), seed + 10 + k),
-                                    S.uniform_queries(mv, n - n // 2, seed + 20 + k)])
-                dq = torch.from_numpy(q).to(dev)
+            if kind == "points":
+                n = len(a)
                 out = torch.empty(n, dtype=torch.float32, device=dev)
-                self.reqs.append(("points", mi, dq, out, n))
+                self.reqs.append(("points", mi, torch.from_numpy(a).to(dev), out, n))
                 self.point_samples += n
+                self.point_mesh_bytes += 12 * len(mv) + 12 * len(mf)
             else:
-                e = int(round(math.exp(g.uniform(math.log(32), math.log(128)))))
-                first, size, count = S.grid_for(mv, (e, e, e))
-                out = torch.empty(e * e * e, dtype=torch.float32, device=dev)
+                first, size, count = a
+                n = count[0] * count[1] * count[2]
+                out = torch.empty(n, dtype=torch.float32, device=dev)
                 grid = ((ctypes.c_float * 3)(*first), (ctypes.c_float * 3)(*size), (ctypes.c_uint64 * 3)(*count))
-                self.reqs.append(("grid", mi, grid, out, e * e * e))
-                self.grid_samples += e * e * e
+                self.reqs.append(("grid", mi, grid, out, n))
+                self.grid_samples += n
+                self.grid_mesh_bytes += 12 * len(mv) + 12 * len(mf)
         torch.cuda.synchronize()
         self.samples = self.point_samples + self.grid_samples
 
@@ -525,6 +541,86 @@ class ServeWorkload:
         return out
 
 
+def host_cpus():
+    """(threads the CPU baseline uses, host details). The threads are the CPUs
+    this process may run on, limited by OMP_NUM_THREADS when the machine sets
+    it (a GPU machine sets it to the CPU share it gives one GPU)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = max(1, min(avail, int(share))) if share.isdigit() and int(share) > 0 else avail
+    model = governor = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cpufreq/scaling_governor") as fh:
+            governor = fh.read().strip()
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity_cpus": avail, "omp_num_threads": share or None,
+                     "threads_used": threads, "cpu_model": model, "governor": governor}
+
+
+def serve_accuracy(sv, O, S, np, threads, budget_pairs=3e9):
+    """Accuracy of the served answers on a bounded sample, against the oracle
+    and an FP64 brute-force distance (Ericson's closest point in double over
+    every triangle). Points: the first point requests' answers (a prefix of
+    each, so the FP64 work stays within budget_pairs point-triangle pairs)
+    against the oracle's Bvh(Raycast). Grid: the grid request with the fewest
+    cell x triangle pairs against the reference's single-thread heap schedule
+    restated by the oracle. Samples whose FP64 distance is within 1e-5 x bbox
+    diagonal of the surface are left out of the sign comparison and counted."""
+    pts = [r for r in sv.reqs if r[0] == "points"][:4]
+    per_req = budget_pairs / 2 / max(1, len(pts))
+    gpu_e, cpu_e, near, sign_bad, n_tot, dmax = [], [], 0, 0, 0, 0.0
+    for kind, mi, dq, out, n in pts:
+        mv, mf = sv.meshes[mi][0], sv.meshes[mi][1]
+        k = int(max(1, min(n, per_req / len(mf))))
+        q = dq[:k].cpu().numpy()
+        gpu = out[:k].cpu().numpy()
+        s, ref = O.generate_sdf(mv, mf, O.TOPO_LIST, q, O.ACCEL_BVH, O.SIGN_RAYCAST, threads)
+        d64 = O.fp64_distance(mv, mf, q, threads)
+        diag = S.bbox_diagonal(mv)
+        far = d64 > 1e-5 * diag
+        near += int(np.sum(~far))
+        sign_bad += int(np.sum(np.signbit(gpu[far]) != np.signbit(ref[far])))
+        dmax = max(dmax, float(np.max(np.abs(gpu - ref)) / diag))
+        gpu_e.append(np.abs(np.abs(gpu).astype(np.float64) - d64) / diag)
+        cpu_e.append(np.abs(np.abs(ref).astype(np.float64) - d64) / diag)
+        n_tot += k
+    ge, ce = np.concatenate(gpu_e), np.concatenate(cpu_e)
+    points = {"samples": n_tot, "requests": len(pts), "reference": "oracle Bvh(Raycast) (generic/bvh.rs)",
+              "sign_mismatches_excluding_near_surface": sign_bad, "near_surface_excluded": near,
+              "max_abs_diff_over_diag": dmax,
+              "fp64_error_over_diag": {"gpu_max": float(ge.max()), "gpu_p99": float(np.percentile(ge, 99)),
+                                       "cpu_max": float(ce.max()), "cpu_p99": float(np.percentile(ce, 99))}}
+    grids = [r for r in sv.reqs if r[0] == "grid"]
+    kind, mi, a, out, n = min(grids, key=lambda r: r[4] * len(sv.meshes[r[1]][1]))
+    mv, mf = sv.meshes[mi][0], sv.meshes[mi][1]
+    first, size, count = np.array(a[0][:], np.float32), np.array(a[1][:], np.float32), tuple(int(c) for c in a[2])
+    s, ref, _ = O.generate_grid_sdf(mv, mf, O.TOPO_LIST, first, size, count, O.SIGN_RAYCAST, 1, O.GRID_SEQUENTIAL)
+    gpu = out.cpu().numpy()
+    cells = O.grid_centres(first, size, count)
+    d64 = O.fp64_distance(mv, mf, cells, threads)
+    diag = S.bbox_diagonal(mv)
+    far = d64 > 1e-5 * diag
+    ge = np.abs(np.abs(gpu).astype(np.float64) - d64) / diag
+    ce = np.abs(np.abs(ref).astype(np.float64) - d64) / diag
+    dd = np.abs(gpu - ref) / diag
+    grid = {"cells": int(n), "triangles": int(len(mf)),
+            "reference": "oracle restatement of generate/grid.rs, 1-thread heap schedule",
+            "sign_mismatches_excluding_near_surface": int(np.sum(np.signbit(gpu[far]) != np.signbit(ref[far]))),
+            "near_surface_excluded": int(np.sum(~far)),
+            "cells_beyond_1e-5_diag_vs_reference": int(np.sum(dd > 1e-5)), "max_abs_diff_over_diag": float(dd.max()),
+            "fp64_error_over_diag": {"gpu_max": float(ge.max()), "gpu_p99": float(np.percentile(ge, 99)),
+                                     "cpu_max": float(ce.max()), "cpu_p99": float(np.percentile(ce, 99)),
+                                     "gpu_cells_beyond_1e-5": int(np.sum(ge > 1e-5)),
+                                     "cpu_cells_beyond_1e-5": int(np.sum(ce > 1e-5))}}
+    return {"points": points, "grid": grid}
+
+
 def cpu_baseline(workload, v, f, env):
     """The oracle's restatement of the reference's CPU algorithm on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -532,7 +628,7 @@ def cpu_baseline(workload, v, f, env):
     import oracle as O
     from mesh_to_sdf_amd import synthetic as S
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    threads, host = host_cpus()
     budget_s, times = 30.0, []
     if workload == "grid":
         first, size, count = env["first"], env["size"], env["count"]
@@ -603,7 +699,8 @@ def cpu_baseline(workload, v, f, env):
         return {"value": done / t, "unit": "samples/s", "cores": threads, "kind": "port",
                 "sample": f"the first {used} of the {len(sv.reqs)} requests ({done} samples) through the oracle's "
                           f"restatement of generic/bvh.rs (points) and generate/grid.rs (grids), {threads} threads",
-                "seconds": t, "speedup": None}
+                "seconds": t, "speedup": None, "host": host,
+                "accuracy": serve_accuracy(sv, O, S, np, threads)}
     if workload == "rtree_bvh":
         q = env["q"]
         t_start = time.perf_counter()
