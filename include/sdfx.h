@@ -167,6 +167,18 @@ sdfx_status sdfx_build_sdf_acceleration(const float* vertices, uint64_t vertex_c
 sdfx_status sdfx_query_sdf(const sdfx_mesh* mesh, const float* query_points, uint64_t query_count, float* out,
                            uint64_t* out_count);
 
+/* sdfx_query_sdf on device memory with no allocation at all (no counterpart
+ * in the reference: for callers that cannot allocate device memory, such as
+ * code under HIP graph capture or serving loops that forbid it). The call's
+ * temporaries come from a fixed 16 MiB scratch array that each GPU holds
+ * while the library is loaded; at most 262,144 queries per call (more:
+ * SDFX_ERR_INVALID_ARGUMENT). Points and outputs are device pointers, work is
+ * enqueued on `stream` (a hipStream_t, NULL for the null stream), and the call
+ * returns when it is done. Calls on one device share that scratch, so the
+ * library runs them one after another. */
+sdfx_status sdfx_query_sdf_noalloc(const sdfx_mesh* mesh, const float* d_query_points, uint64_t query_count,
+                                   float* d_out, uint64_t* out_count, void* stream);
+
 /* Replaces generic/rtree_bvh.rs:217-276 query_vector_to_closest_point_rtree_bvh
  * (re-exported at lib.rs:150). `mesh` must have been built with
  * SDFX_ACCEL_RTREE_BVH. vectors_out holds 3*query_count floats (closest - query),

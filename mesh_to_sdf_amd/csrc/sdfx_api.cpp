@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <sstream>
 #include <vector>
@@ -495,6 +496,76 @@ sdfx_status sdfx_query_sdf(const sdfx_mesh* mesh, const float* query_points, uin
         }
         SDFX_HIP(hipMemcpyAsync(out, dout.get(), query_count * sizeof(float), hipMemcpyDeviceToHost, s));
         SDFX_HIP(hipStreamSynchronize(s));
+        if (out_count) *out_count = query_count;
+        return SDFX_OK;
+    });
+}
+
+// ---- sdfx_query_sdf_noalloc: sdfx_query_sdf on device memory without any
+// allocation. Its temporaries (error flags, the Morton order of the queries
+// and its radix-sort buffers, the deferred-search records) come from
+// `scratch`, a device array each GPU holds from the library's load to the end
+// of the process. In host code `scratch` names a host-side stand-in; its
+// device address on the current device comes from hipGetSymbolAddress, once
+// per device, and only that address is used.
+
+namespace {
+
+constexpr uint64_t NOALLOC_MAX_QUERIES = 262144;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16u << 20;  // 48 B per query at most, plus small tables
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+struct NoallocDevice {
+    std::mutex mu;  // held for a whole call: calls on one device run one after another
+    void* addr = nullptr;
+};
+NoallocDevice g_noalloc[NOALLOC_MAX_DEVICES];
+
+// The device address of `scratch` on the current device `dev`, looked up on
+// the first call on that device.
+void* noalloc_scratch(int dev) {
+    NoallocDevice& D = g_noalloc[dev];
+    if (!D.addr) {
+        void* p = nullptr;
+        SDFX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(SDFX_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        D.addr = p;
+    }
+    return D.addr;
+}
+
+}  // namespace
+
+sdfx_status sdfx_query_sdf_noalloc(const sdfx_mesh* mesh, const float* d_query_points, uint64_t query_count,
+                                   float* d_out, uint64_t* out_count, void* stream) {
+    return guarded([&]() -> sdfx_status {
+        require(mesh != nullptr, "mesh");
+        require(d_query_points || query_count == 0, "query_points");
+        require(d_out || query_count == 0, "out");
+        if (out_count) *out_count = 0;
+        if (query_count > NOALLOC_MAX_QUERIES)
+            fail(SDFX_ERR_INVALID_ARGUMENT, "sdfx_query_sdf_noalloc takes at most " +
+                                                std::to_string(NOALLOC_MAX_QUERIES) + " queries per call (got " +
+                                                std::to_string(query_count) + ")");
+        if (query_count == 0) return SDFX_OK;
+        if (mesh->deferred != SDFX_OK) fail(mesh->deferred, mesh->deferred_msg);
+        if (mesh->method != SDFX_ACCEL_NONE && mesh->n_tris == 0) fail_empty_mesh();
+        if (mesh->device < 0 || mesh->device >= NOALLOC_MAX_DEVICES) fail(SDFX_ERR_INVALID_ARGUMENT, "device index");
+        SDFX_HIP(hipSetDevice(mesh->device));
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        std::lock_guard<std::mutex> lk(g_noalloc[mesh->device].mu);
+        void* base = noalloc_scratch(mesh->device);
+        Arena A(s, base, NOALLOC_SCRATCH_BYTES);
+        DevFlags* flags = new_flags(A);
+        if (mesh->method == SDFX_ACCEL_NONE)
+            none_query(mesh->tris.get(), mesh->n_tris, d_query_points, query_count, mesh->sign, d_out, flags, A);
+        else
+            lbvh_query(mesh->bvh, d_query_points, query_count, mode_for(mesh->method, mesh->sign), d_out, flags, A);
+        DevFlags f = read_flags(flags, s);
+        check_internal(f);
+        if (f.nan_distance) fail_nan();
         if (out_count) *out_count = query_count;
         return SDFX_OK;
     });

@@ -191,6 +191,7 @@ void pinned_free(void* p) {
 
 void Arena::upload(void* d_dst, const void* h_src, size_t bytes) {
     if (!bytes) return;
+    if (region) throw Error(SDFX_ERR_INVALID_ARGUMENT, "host uploads need pinned staging memory, which a call on a fixed region cannot allocate");
     void* h = pinned_alloc(bytes);
     staged.push_back(h);
     std::memcpy(h, h_src, bytes);

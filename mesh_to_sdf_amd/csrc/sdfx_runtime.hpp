@@ -93,16 +93,32 @@ struct DevBuf {
 // the stream is synchronised (in the destructor), so an exception thrown while
 // kernels are in flight cannot hand their memory to another call. Declare the
 // Arena after every DevBuf a call owns, so it is destroyed (and syncs) first.
+//
+// An Arena built over a fixed region (the scratch of a no-allocation entry
+// point) hands out consecutive 256-byte-aligned pieces of it instead, never
+// calls an allocator, and fails a request beyond its end; upload(), which
+// needs pinned staging memory, is unavailable there.
 struct Arena {
     hipStream_t stream;
     std::vector<void*> blocks;
     std::vector<void*> staged;  // pinned host blocks of upload()
+    char* region = nullptr;     // fixed region, or null
+    size_t region_bytes = 0, region_used = 0;
     explicit Arena(hipStream_t s) : stream(s) {}
+    Arena(hipStream_t s, void* base, size_t bytes) : stream(s), region(static_cast<char*>(base)), region_bytes(bytes) {}
     Arena(const Arena&) = delete;
     Arena& operator=(const Arena&) = delete;
     template <typename T>
     T* alloc(size_t count) {
-        void* p = dev_alloc((count ? count : 1) * sizeof(T));
+        const size_t bytes = (count ? count : 1) * sizeof(T);
+        if (region) {
+            const size_t off = (region_used + 255) & ~size_t(255);
+            if (off > region_bytes || bytes > region_bytes - off)
+                throw Error(SDFX_ERR_INVALID_ARGUMENT, "the call needs more scratch than the fixed region holds");
+            region_used = off + bytes;
+            return reinterpret_cast<T*>(region + off);
+        }
+        void* p = dev_alloc(bytes);
         blocks.push_back(p);
         return static_cast<T*>(p);
     }
