@@ -272,17 +272,27 @@ sdfx_status grid_batch_core(sdfx_grid_request* reqs, uint64_t n, sdfx_sign_metho
     fill_u32(reinterpret_cast<uint32_t*>(flags), (n ? n : 1) * (sizeof(DevFlags) / sizeof(uint32_t)), 0u, s);
     uint64_t off = 0;
     std::vector<GatherJob> gathers;
+    std::vector<uint64_t> item_req;  // request of each item
     for (uint64_t i = 0; i < n; ++i) {
         if (prep[i].status != SDFX_OK) continue;
         const sdfx_grid_request& r = reqs[i];
         gathers.push_back(GatherJob{r.vertices, r.vertex_count, r.indices, r.topology, prep[i].nt, tris.get() + off,
                                     flags + i});
-        if (!(prep[i].zero_axis && prep[i].nt > 0))
+        if (!(prep[i].zero_axis && prep[i].nt > 0)) {
             items.push_back(GridBatchItem{tris.get() + off, off, prep[i].nt, prep[i].g, r.out, (uint32_t)i});
+            item_req.push_back(i);
+        }
         off += prep[i].nt;
     }
     gather_triangles_batch(gathers, A);
-    grid_sdf_batch(items, sign, flags, A);
+    // a run that fails (its scratch, its launches) fails only its own requests
+    std::vector<std::pair<sdfx_status, std::string>> run_errors;
+    grid_sdf_batch(items, sign, flags, A, &run_errors);
+    for (size_t k = 0; k < items.size(); ++k)
+        if (run_errors[k].first != SDFX_OK) {
+            prep[item_req[k]].status = run_errors[k].first;
+            prep[item_req[k]].message = run_errors[k].second;
+        }
     std::vector<DevFlags> hf(n ? n : 1);
     SDFX_HIP(hipMemcpyAsync(hf.data(), flags, (n ? n : 1) * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
     SDFX_HIP(hipStreamSynchronize(s));
@@ -640,12 +650,17 @@ sdfx_status sdfx_generate_grid_sdf_batch(sdfx_grid_request* requests, uint64_t r
         std::vector<DevBuf<float>> dv(request_count), dout(request_count);
         std::vector<DevBuf<uint32_t>> di(request_count);
         std::vector<sdfx_grid_request> dreq(requests, requests + request_count);
+        // the host-side checks first, so only valid requests' buffers are
+        // allocated and uploaded (a failing request keeps its status)
+        std::vector<uint64_t> ncells(request_count, 0);
         for (uint64_t i = 0; i < request_count; ++i) {
             const sdfx_grid_request& r = requests[i];
-            const uint64_t nc = r.cell_count[0] * r.cell_count[1] * r.cell_count[2];
+            const BatchPrep p = prepare_grid_request(r, sign);
+            if (p.status != SDFX_OK) continue;  // grid_batch_core reports it
+            if (!p.zero_axis) ncells[i] = r.cell_count[0] * r.cell_count[1] * r.cell_count[2];
             if (r.vertices) upload(dv[i], r.vertices, 3 * r.vertex_count, s);
             if (r.indices) upload(di[i], r.indices, r.index_count, s);
-            dout[i].reset(nc);
+            dout[i].reset(ncells[i]);
             dreq[i].vertices = r.vertices ? dv[i].get() : nullptr;
             dreq[i].indices = r.indices ? di[i].get() : nullptr;
             dreq[i].out = r.out ? dout[i].get() : nullptr;
@@ -653,9 +668,9 @@ sdfx_status sdfx_generate_grid_sdf_batch(sdfx_grid_request* requests, uint64_t r
         const sdfx_status st = grid_batch_core(dreq.data(), request_count, sign, s);
         for (uint64_t i = 0; i < request_count; ++i) {
             requests[i].status = dreq[i].status;
-            const uint64_t nc = requests[i].cell_count[0] * requests[i].cell_count[1] * requests[i].cell_count[2];
-            if (dreq[i].status == SDFX_OK && nc && requests[i].out)
-                SDFX_HIP(hipMemcpyAsync(requests[i].out, dout[i].get(), nc * sizeof(float), hipMemcpyDeviceToHost, s));
+            if (dreq[i].status == SDFX_OK && ncells[i] && requests[i].out)
+                SDFX_HIP(hipMemcpyAsync(requests[i].out, dout[i].get(), ncells[i] * sizeof(float),
+                                        hipMemcpyDeviceToHost, s));
         }
         SDFX_HIP(hipStreamSynchronize(s));
         return st;

@@ -42,8 +42,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <utility>
 #include <vector>
 
+#include "sdfx_grid_plan.hpp"
 #include "sdfx_internal.hpp"
 
 namespace sdfx {
@@ -639,7 +642,7 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
             if (total) atomicAdd(req_word<MODE>(taken_cur, q, tile_id), total);
         }
         if (take) {
-            uint32_t slot = cls * region + block_base + before + x - mine;
+            uint64_t slot = (uint64_t)cls * region + block_base + before + x - mine;
 #pragma unroll
             for (int k = 0; k < COMPACT_PER_LANE; ++k) {
                 if (!((take >> k) & 1u)) continue;
@@ -662,6 +665,9 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
 // it holds that many, so the distance evaluations run with every lane busy
 // instead of idling the lanes whose candidate was skipped.
 constexpr uint32_t PROP_QUEUE = 128;  // a fill adds at most 64 to at most 63 left over
+// Frontier items one propagation launch takes: 26 neighbour slots each plus
+// one grid stride stay below 2^32.
+constexpr uint32_t PROP_CHUNK_ITEMS = (uint32_t)(((1ull << 32) - (1ull << 24)) / 26);
 constexpr uint32_t PROP_WAVES = 4;    // 256-thread blocks
 
 template <bool BATCH>
@@ -780,6 +786,7 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
                                                             uint32_t* __restrict__ taken_next,
                                                             uint32_t* __restrict__ total_out,
                                                             uint32_t* __restrict__ host_probe, uint32_t host_seq,
+                                                            uint32_t item_base,
                                                             DevFlags* flags,
                                                             unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
@@ -811,7 +818,12 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
         taken_next[w * WORD_STRIDE] = 0u;
     }
     __syncthreads();
-    const uint32_t work = prefix[NCLASS] * 26u;  // below 2^32: the frontier holds fewer than 2^32 / 26 cells
+    // this launch's items: [item_base, item_base + PROP_CHUNK_ITEMS) of the
+    // frontier (one launch covers the frontier of grids below 165M cells;
+    // larger ones run several launches per round), so the 32-bit work index
+    // (26 neighbour slots per item) never overflows
+    const uint32_t nfront = prefix[NCLASS];
+    const uint32_t work = nfront > item_base ? min(nfront - item_base, PROP_CHUNK_ITEMS) * 26u : 0u;
     // Each XCD (blocks b with b % XCDS == x) takes one contiguous eighth of the
     // items, which lie in spatial order, so keys and triangles of neighbouring
     // cells share that XCD's L2.
@@ -827,7 +839,7 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
         uint32_t x = 0, y = 0, z = 0, tri = 0, q = 0;
         uint64_t cur = 0;
         if (j < j1) {
-            const uint32_t f = j / 26u;
+            const uint32_t f = item_base + j / 26u;
             uint32_t k = j - f * 26u;
             k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
             uint32_t cls = 0;  // the last class whose prefix is <= f (a non-empty one)
@@ -835,7 +847,7 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
             for (uint32_t step = NCLASS / 2; step; step >>= 1)
                 if (prefix[cls + step] <= f) cls += step;
             uint32_t ix, iy, iz;
-            read_item<MODE>(items[cls * region + (f - prefix[cls])], tri, q, ix, iy, iz);
+            read_item<MODE>(items[(uint64_t)cls * region + (f - prefix[cls])], tri, q, ix, iy, iz);
             x = ix + k / 9u - 1u;
             y = iy + (k / 3u) % 3u - 1u;
             z = iz + k % 3u - 1u;
@@ -1199,11 +1211,10 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         const uint64_t tiles = (nc + COMPACT_TILE - 1) / COMPACT_TILE;
         ncells += tiles * COMPACT_TILE;
         max_extent = std::max<uint64_t>(max_extent, J.gp.count[0] + J.gp.count[1] + J.gp.count[2]);
-        // 32-bit cell and work indices: 26 neighbour slots per cell plus one
-        // grid stride stay below 2^32
-        if (ncells * 26 + (1ull << 24) >= (1ull << 32))
-            throw Error(SDFX_ERR_INVALID_ARGUMENT, "grids are limited to 165,000,000 cells per launch");
     }
+    // frontier counts and cell indices within a request are 32-bit; a run
+    // holds fewer than 2^32 tile-padded cells (grid_sdf_batch splits batches)
+    if (ncells > (1ull << 32)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "a grid run holds at most 2^32 cells");
     const uint32_t ntiles = (uint32_t)(ncells / COMPACT_TILE);
     // a single request travels in the kernel arguments; a batch gets its table,
     // and the owner request of every tile and triangle, on the device
@@ -1301,6 +1312,9 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // propagation's loops are grid-strided, so a round that grows again is
     // still complete.
     uint32_t last_taken = UINT32_MAX;
+    // a round's frontier holds at most ncells items; each propagation launch
+    // takes PROP_CHUNK_ITEMS of them (one launch below 165M cells)
+    const uint32_t prop_chunks = (uint32_t)((ncells + PROP_CHUNK_ITEMS - 1) / PROP_CHUNK_ITEMS);
     auto enqueue_batch = [&](int slot) {
         const uint32_t seq = probe.arm(slot);
         const unsigned blocks = last_taken < 2048u ? std::min(prop_blocks, 256u) : prop_blocks;
@@ -1308,10 +1322,11 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
             SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
                         taken[cur ^ 1], taken[cur], released, (int)SIGN);
-            SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(blocks), dim3(256), 0, s, items, region, ctr[cur],
-                        rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
-                        totals + cur, b == batch - 1 ? probe.probe(slot) : nullptr, seq, d_flags,
-                        evals ? evals + seed_slots : nullptr);
+            for (uint32_t k = 0; k < prop_chunks; ++k)
+                SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(blocks), dim3(256), 0, s, items, region, ctr[cur],
+                            rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
+                            totals + cur, b == batch - 1 && k + 1 == prop_chunks ? probe.probe(slot) : nullptr, seq,
+                            k * PROP_CHUNK_ITEMS, d_flags, evals ? evals + seed_slots : nullptr);
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());
@@ -1348,10 +1363,31 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     }
 }
 
-void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method sign, DevFlags* d_flags, Arena& A) {
-    std::vector<GridBatchItem> jobs;
-    uint64_t nt = 0;
-    for (const GridBatchItem& it : items) {
+// Tile-padded cells one run of a batch may hold: a run's device state is about
+// 25 bytes per cell (keys, flags, frontier items), so larger batches run as
+// several runs, each with its own scratch returned before the next starts; a
+// single request beyond it runs alone (up to the 2^32 cells the entry points
+// admit).
+constexpr uint64_t RUN_MAX_CELLS = 1ull << 28;
+
+static uint64_t padded_cells(const GridParams& g) {
+    const uint64_t nc = g.count[0] * g.count[1] * g.count[2];
+    return (nc + COMPACT_TILE - 1) / COMPACT_TILE * COMPACT_TILE;
+}
+
+template <int SIGN>
+static void grid_run_any(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFlags* d_flags, Arena& A) {
+    if (jobs.size() > 1) grid_run<SIGN | 2>(nt, jobs, d_flags, A);
+    else grid_run<SIGN>(nt, jobs, d_flags, A);
+}
+
+void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method sign, DevFlags* d_flags, Arena& A,
+                    std::vector<std::pair<sdfx_status, std::string>>* run_errors) {
+    if (run_errors) run_errors->assign(items.size(), {SDFX_OK, std::string()});
+    // requests with cells and triangles, in order, cut into runs
+    std::vector<uint64_t> cells(items.size(), 0);
+    for (size_t i = 0; i < items.size(); ++i) {
+        const GridBatchItem& it = items[i];
         const uint64_t ncells = it.gp.count[0] * it.gp.count[1] * it.gp.count[2];
         if (!ncells) continue;
         if (it.n_tris == 0) {
@@ -1360,24 +1396,35 @@ void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method si
                         fbits(FLT_MAX));
             continue;
         }
-        jobs.push_back(it);
-        nt = std::max<uint64_t>(nt, it.tri_off + it.n_tris);
+        cells[i] = padded_cells(it.gp);
     }
     SDFX_HIP(hipGetLastError());
-    if (jobs.empty()) return;
-    const bool batch = jobs.size() > 1;
-    if (sign == SDFX_SIGN_NORMAL) {
-        if (batch) grid_run<SDFX_SIGN_NORMAL | 2>(nt, jobs, d_flags, A);
-        else grid_run<SDFX_SIGN_NORMAL>(nt, jobs, d_flags, A);
-    } else {
-        if (batch) grid_run<SDFX_SIGN_RAYCAST | 2>(nt, jobs, d_flags, A);
-        else grid_run<SDFX_SIGN_RAYCAST>(nt, jobs, d_flags, A);
+    const std::vector<std::vector<size_t>> runs = plan_grid_runs(cells, RUN_MAX_CELLS);
+    for (const std::vector<size_t>& run : runs) {
+        // triangle numbering local to the run
+        uint64_t base = UINT64_MAX, nt = 0;
+        for (size_t i : run) base = std::min(base, items[i].tri_off);
+        std::vector<GridBatchItem> jobs;
+        for (size_t i : run) {
+            GridBatchItem j = items[i];
+            j.tri_off -= base;
+            nt = std::max(nt, j.tri_off + j.n_tris);
+            jobs.push_back(j);
+        }
+        try {
+            Arena R(A.stream);  // the run's scratch, returned (after a sync) before the next run
+            if (sign == SDFX_SIGN_NORMAL) grid_run_any<SDFX_SIGN_NORMAL>(nt, jobs, d_flags, R);
+            else grid_run_any<SDFX_SIGN_RAYCAST>(nt, jobs, d_flags, R);
+        } catch (const Error& e) {
+            if (!run_errors) throw;
+            for (size_t i : run) (*run_errors)[i] = {e.status, e.what()};
+        }
     }
 }
 
 void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
               DevFlags* d_flags, Arena& A) {
-    grid_sdf_batch({GridBatchItem{d_tris, 0, nt, gp, d_out, 0}}, sign, d_flags, A);
+    grid_sdf_batch({GridBatchItem{d_tris, 0, nt, gp, d_out, 0}}, sign, d_flags, A, nullptr);
 }
 
 }  // namespace sdfx

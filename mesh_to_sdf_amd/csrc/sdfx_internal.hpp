@@ -177,10 +177,13 @@ struct GridBatchItem {
     float* d_out;
     uint32_t flag;
 };
-// All requests (one sign method) through the same launches; each result is
-// bit-identical to grid_sdf on that request alone.
+// All requests (one sign method) through shared launches, in runs of at most
+// about 2^28 cells each; each result is bit-identical to grid_sdf on that
+// request alone. With run_errors, a run that fails records its error for its
+// requests (run_errors[i] for items[i]) and the others still run; without,
+// the error is thrown.
 void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method sign, DevFlags* d_flags,
-                    Arena& arena);
+                    Arena& arena, std::vector<std::pair<sdfx_status, std::string>>* run_errors);
 
 // ---------------------------------------------------------------- utilities
 

@@ -179,7 +179,7 @@ void sdfx_server::run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, 
             if (!points[q].empty()) lbvh_query_batch(points[q], (QueryMode)q, flags, B);
         SDFX_HIP(hipEventRecord(ws.points_done, ws.points));
         for (int g = 0; g < 2; ++g)
-            if (!grid[g].empty()) grid_sdf_batch(grid[g], g ? SDFX_SIGN_NORMAL : SDFX_SIGN_RAYCAST, flags, A);
+            if (!grid[g].empty()) grid_sdf_batch(grid[g], g ? SDFX_SIGN_NORMAL : SDFX_SIGN_RAYCAST, flags, A, nullptr);
         SDFX_HIP(hipStreamWaitEvent(s, ws.points_done, 0));
         SDFX_HIP(hipMemcpyAsync(hf.data(), flags, n * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
         SDFX_HIP(hipStreamSynchronize(s));
