@@ -179,6 +179,27 @@ sdfx_status sdfx_query_sdf(const sdfx_mesh* mesh, const float* query_points, uin
 sdfx_status sdfx_query_sdf_noalloc(const sdfx_mesh* mesh, const float* d_query_points, uint64_t query_count,
                                    float* d_out, uint64_t* out_count, void* stream);
 
+/* One request of sdfx_query_sdf_batch_device: a resident mesh, its query
+ * points and the output (device pointers); `status` is set by the call. */
+typedef struct sdfx_point_request {
+    const sdfx_mesh* mesh;
+    const float* query_points;
+    uint64_t query_count;
+    float* out;
+    sdfx_status status;
+} sdfx_point_request;
+
+/* query_sdf (lib.rs:463-485) for many resident meshes with uneven query
+ * counts in one call (no counterpart in the reference, which answers one mesh
+ * per call; for callers with many meshes): the requests of each query mode
+ * share the same LBVH launches (queries sorted by request, then Morton code),
+ * and each output is bit-identical to sdfx_query_sdf on that request alone.
+ * All meshes must live on one device. Returns SDFX_OK when every request
+ * succeeded, else the first failing request's status (each request's own
+ * status is in its `status`; sdfx_last_error names the request). Work is
+ * enqueued on `stream` and the call returns when it is done. */
+sdfx_status sdfx_query_sdf_batch_device(sdfx_point_request* requests, uint64_t request_count, void* stream);
+
 /* Replaces generic/rtree_bvh.rs:217-276 query_vector_to_closest_point_rtree_bvh
  * (re-exported at lib.rs:150). `mesh` must have been built with
  * SDFX_ACCEL_RTREE_BVH. vectors_out holds 3*query_count floats (closest - query),

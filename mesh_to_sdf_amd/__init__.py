@@ -31,7 +31,7 @@ __all__ = [
     "SignMethod", "AccelerationMethod", "ALL_SDF_ACCELERATION_METHODS", "Topology", "Grid", "SnapResult",
     "generate_sdf", "generate_grid_sdf", "build_sdf_acceleration", "query_sdf",
     "query_vector_to_closest_point_rtree_bvh", "SdfAccelerationMesh", "SdfxError", "check_mesh_triangle_list",
-    "generate_grid_sdf_batch",
+    "generate_grid_sdf_batch", "query_sdf_batch",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -331,6 +331,37 @@ def query_sdf(acceleration: SdfAccelerationMesh, query_points) -> np.ndarray:
     n = ctypes.c_uint64(0)
     _lib.check(_lib.lib().sdfx_query_sdf(acceleration._h, _ptr(q, _F), len(q), _ptr(out, _F), ctypes.byref(n)))
     return out[: n.value]
+
+
+def query_sdf_batch(requests, return_status: bool = False):
+    """query_sdf (lib.rs:463-485) for many (SdfAccelerationMesh, query_points)
+    pairs in one call (sdfx_query_sdf_batch_device): requests of one query
+    mode share the same launches and each result is bit-identical to query_sdf
+    on that pair alone. Points go to the device and results come back as
+    numpy arrays. Raises the first failing request's error, or with
+    return_status=True returns (results, statuses) with None for failed
+    requests."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    reqs = (_lib.PointRequest * max(1, len(requests)))()
+    keep = []
+    for r, (acc, pts) in zip(reqs, requests):
+        q = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)).to(dev)
+        out = torch.empty(q.shape[0], dtype=torch.float32, device=dev)
+        keep.append((q, out))
+        r.mesh = acc._h.value
+        r.query_points = q.data_ptr() if q.numel() else None
+        r.query_count = q.shape[0]
+        r.out = out.data_ptr() if out.numel() else None
+    stream = torch.cuda.current_stream()
+    stream.synchronize()
+    st = _lib.lib().sdfx_query_sdf_batch_device(reqs, len(requests), ctypes.c_void_p(stream.cuda_stream))
+    statuses = [reqs[i].status for i in range(len(requests))]
+    outs = [out.cpu().numpy() if s_ == _lib.OK else None for (q, out), s_ in zip(keep, statuses)]
+    if return_status:
+        return outs, statuses
+    _lib.check(st)
+    return outs
 
 
 def query_vector_to_closest_point_rtree_bvh(acceleration: SdfAccelerationMesh, query_points):

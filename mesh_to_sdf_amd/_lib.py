@@ -52,6 +52,12 @@ class ServerMesh(ctypes.Structure):
                 ("status", ctypes.c_int)]
 
 
+class PointRequest(ctypes.Structure):
+    """sdfx_point_request (include/sdfx.h)."""
+    _fields_ = [("mesh", ctypes.c_void_p), ("query_points", ctypes.c_void_p), ("query_count", ctypes.c_uint64),
+                ("out", ctypes.c_void_p), ("status", ctypes.c_int)]
+
+
 class ServerStats(ctypes.Structure):
     """sdfx_server_stats (include/sdfx.h)."""
     _fields_ = [(n, ctypes.c_uint64) for n in ("requests_done", "batches", "max_batch_requests", "resident_bytes",
@@ -69,6 +75,7 @@ SIGNATURES = {
     "sdfx_build_sdf_acceleration": (_i, [_f, _u64v, _u32, _u64v, _i, _i, _i, ctypes.POINTER(_vp)]),
     "sdfx_query_sdf": (_i, [_vp, _f, _u64v, _f, _u64]),
     "sdfx_query_sdf_noalloc": (_i, [_vp, _vp, _u64v, _vp, _u64, _vp]),
+    "sdfx_query_sdf_batch_device": (_i, [ctypes.POINTER(PointRequest), _u64v, _vp]),
     "sdfx_query_vector_to_closest_point": (_i, [_vp, _f, _u64v, _f, _u8]),
     "sdfx_free_sdf_acceleration": (None, [_vp]),
     "sdfx_generate_grid_sdf_batch": (_i, [ctypes.POINTER(GridRequest), _u64v, _i]),

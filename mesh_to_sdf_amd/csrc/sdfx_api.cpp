@@ -581,6 +581,72 @@ sdfx_status sdfx_query_sdf_noalloc(const sdfx_mesh* mesh, const float* d_query_p
     });
 }
 
+sdfx_status sdfx_query_sdf_batch_device(sdfx_point_request* requests, uint64_t request_count, void* stream) {
+    return guarded([&]() -> sdfx_status {
+        require(requests || request_count == 0, "requests");
+        if (request_count == 0) return SDFX_OK;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // sdfx_query_sdf's host-side checks, per request
+        std::vector<std::string> msg(request_count);
+        int device = -1;
+        for (uint64_t i = 0; i < request_count; ++i) {
+            sdfx_point_request& r = requests[i];
+            r.status = SDFX_OK;
+            try {
+                require(r.mesh != nullptr, "mesh");
+                require(r.query_points || r.query_count == 0, "query_points");
+                require(r.out || r.query_count == 0, "out");
+                if (r.query_count == 0) continue;
+                if (r.mesh->deferred != SDFX_OK) fail(r.mesh->deferred, r.mesh->deferred_msg);
+                if (r.mesh->method != SDFX_ACCEL_NONE && r.mesh->n_tris == 0) fail_empty_mesh();
+                if (device < 0) device = r.mesh->device;
+                if (r.mesh->device != device) fail(SDFX_ERR_INVALID_ARGUMENT, "the meshes of a batch live on one device");
+            } catch (const Error& e) {
+                r.status = e.status;
+                msg[i] = e.what();
+            }
+        }
+        if (device >= 0) {
+            SDFX_HIP(hipSetDevice(device));
+            Arena A(s);
+            DevFlags* flags = A.alloc<DevFlags>(request_count);
+            fill_u32(reinterpret_cast<uint32_t*>(flags), request_count * (sizeof(DevFlags) / sizeof(uint32_t)), 0u, s);
+            std::vector<PointBatchItem> by_mode[4];
+            for (uint64_t i = 0; i < request_count; ++i) {
+                const sdfx_point_request& r = requests[i];
+                if (r.status != SDFX_OK || r.query_count == 0) continue;
+                if (r.mesh->method == SDFX_ACCEL_NONE)
+                    none_query(r.mesh->tris.get(), r.mesh->n_tris, r.query_points, r.query_count, r.mesh->sign, r.out,
+                               flags + i, A);
+                else
+                    by_mode[mode_for(r.mesh->method, r.mesh->sign)].push_back(
+                        PointBatchItem{&r.mesh->bvh, r.query_points, r.query_count, r.out, (uint32_t)i});
+            }
+            for (int m = 0; m < 4; ++m)
+                if (!by_mode[m].empty()) lbvh_query_batch(by_mode[m], (QueryMode)m, flags, A);
+            std::vector<DevFlags> hf(request_count);
+            SDFX_HIP(hipMemcpyAsync(hf.data(), flags, request_count * sizeof(DevFlags), hipMemcpyDeviceToHost, s));
+            SDFX_HIP(hipStreamSynchronize(s));
+            for (uint64_t i = 0; i < request_count; ++i) {
+                if (requests[i].status != SDFX_OK || requests[i].query_count == 0) continue;
+                if (hf[i].internal) {
+                    requests[i].status = SDFX_ERR_DEVICE;
+                    msg[i] = "device consistency check failed (traversal stack or work decode)";
+                } else if (hf[i].nan_distance) {
+                    requests[i].status = SDFX_ERR_NAN_DISTANCE;
+                    msg[i] = "NaN distance";
+                }
+            }
+        }
+        for (uint64_t i = 0; i < request_count; ++i)
+            if (requests[i].status != SDFX_OK) {
+                set_last_error("request " + std::to_string(i) + ": " + msg[i]);
+                return requests[i].status;
+            }
+        return SDFX_OK;
+    });
+}
+
 sdfx_status sdfx_query_vector_to_closest_point(const sdfx_mesh* mesh, const float* query_points,
                                                uint64_t query_count, float* vectors_out, uint8_t* inside_out) {
     return guarded([&]() -> sdfx_status {
