@@ -96,6 +96,9 @@ struct sdfx_server {
     std::deque<std::shared_ptr<ServerRequest>> queue;
     std::unordered_map<uint64_t, std::shared_ptr<ServerRequest>> tickets;
     std::unordered_map<uint32_t, std::shared_ptr<ServerMesh>> meshes;
+    // Removed meshes that queued or running requests still hold: their bytes
+    // stay in `resident` until the last such request lets go (release_retired).
+    std::vector<std::shared_ptr<ServerMesh>> retired;
     std::vector<std::thread> workers;
     bool stop = false;
     uint64_t next_ticket = 1;
@@ -140,6 +143,20 @@ struct sdfx_server {
         hipEvent_t flags_ready = nullptr, points_done = nullptr;
     };
     void run_batch(std::vector<std::shared_ptr<ServerRequest>>& batch, const WorkerStreams& ws);
+    // Drops the retired meshes no request holds any more and returns their
+    // bytes to the budget (with `mu` held).
+    void release_retired() {
+        for (size_t k = 0; k < retired.size();) {
+            if (retired[k].use_count() == 1) {
+                resident -= retired[k]->bytes;
+                stats.resident_bytes = resident;
+                retired[k] = std::move(retired.back());
+                retired.pop_back();
+            } else {
+                ++k;
+            }
+        }
+    }
 };
 
 // A batch's point requests (None and LBVH) run on the worker's point stream
@@ -245,7 +262,9 @@ void sdfx_server::worker_main() {
             }
             r->t_done = t;
             r->done = true;
+            r->mesh.reset();  // a removed mesh is freed once no request holds it
         }
+        release_retired();
         inflight -= bytes;
         stats.inflight_bytes = inflight;
         stats.requests_done += batch.size();
@@ -496,9 +515,10 @@ sdfx_status sdfx_server_remove_mesh(sdfx_server* sv, uint32_t mesh_id) {
         std::lock_guard<std::mutex> lk(sv->mu);
         auto m = find_mesh(sv, mesh_id);
         sv->meshes.erase(mesh_id);
-        // queued and running requests hold the mesh until they finish
-        sv->resident -= m->bytes;
-        sv->stats.resident_bytes = sv->resident;
+        // queued and running requests hold the mesh until they finish: its
+        // bytes leave the budget when the last of them does
+        sv->retired.push_back(std::move(m));
+        sv->release_retired();
         sv->stats.meshes = sv->meshes.size();
         sv->cv_work.notify_all();
         return SDFX_OK;
@@ -617,6 +637,8 @@ sdfx_status sdfx_server_cancel(sdfx_server* sv, uint64_t ticket, int* cancelled_
             auto q = std::find(sv->queue.begin(), sv->queue.end(), r);
             if (q == sv->queue.end()) return SDFX_OK;  // running or done
             sv->queue.erase(q);
+            r->mesh.reset();
+            sv->release_retired();
             r->status = SDFX_ERR_CANCELLED;
             r->message = "request cancelled";
             r->t_done = Clock::now();

@@ -314,3 +314,28 @@ def test_concurrent_clients(sdf):
         assert not errors, errors
         st = srv.stats()
         assert st["requests_done"] == 36 and st["queued"] == 0 and st["inflight_bytes"] == 0
+
+
+def test_removed_mesh_stays_counted_until_its_requests_finish(sdf):
+    """A mesh removed while a request for it waits behind a long grid keeps
+    its bytes in the budget (its buffers are still in use) until that request
+    is done; the request is answered as if the mesh were still there."""
+    big_v, big_f = S.torus(1.0, 0.35, 128, 64, 0.002, seed=3)
+    v, f = S.noise_sphere(4, 0.05, seed=7)
+    q = S.uniform_queries(v, 50_000, 8)
+    method = sdf.AccelerationMethod.Bvh(sdf.SignMethod.Raycast)
+    with _server(workers=1) as srv:
+        m0 = srv.add_mesh(big_v, sdf.Topology.TriangleList(big_f))
+        r0 = srv.stats()["resident_bytes"]
+        long_grid = srv.submit_grid(m0, sdf.Grid(*S.grid_for(big_v, (256, 256, 256))), 0)
+        m1 = srv.add_mesh(v, sdf.Topology.TriangleList(f))
+        r1 = srv.stats()["resident_bytes"]
+        tk = srv.submit_points(m1, q, method)
+        srv.remove_mesh(m1)
+        st = srv.stats()
+        assert st["meshes"] == 1 and st["resident_bytes"] == r1 > r0, st
+        got, _ = srv.wait(tk)
+        srv.wait(long_grid)
+        assert srv.stats()["resident_bytes"] == r0
+    alone = sdf.generate_sdf(v, sdf.Topology.TriangleList(f), q, method)
+    assert np.array_equal(bits(got.cpu().numpy()), bits(alone))
