@@ -924,58 +924,149 @@ __device__ __forceinline__ void row_range(const GridDev& g, int k, float lo, flo
     i1 = (uint32_t)e;
 }
 
+// The ray rows of triangle t along axis a (work item 3 t + a): the rows whose
+// origin lies in the triangle's padded box, a0..a0+na-1 along the first
+// perpendicular axis and c0..c0+nc-1 along the second (none when the box lies
+// entirely behind the rows' origins).
+struct RayRows {
+    uint32_t a0, c0, na, nc;
+};
+
 template <int AXIS>
-__device__ void scatter_axis(const TriRec& tr, const Box& b, const GridDev& g, uint32_t* __restrict__ bits) {
+__device__ __forceinline__ RayRows ray_rows(const Box& b, const GridDev& g) {
     constexpr int K1 = AXIS == 0 ? 1 : 0;  // the row index is (K1, K2) with K2 fastest
     constexpr int K2 = AXIS == 2 ? 1 : 2;
-    uint32_t a0, a1, c0, c1;
-    row_range(g, K1, b.lo[K1], b.hi[K1], a0, a1);
-    row_range(g, K2, b.lo[K2], b.hi[K2], c0, c1);
-    const float ox_along = cell_coord(g, AXIS, 0u);
-    for (uint32_t i = a0; i <= a1 && a0 <= a1; ++i) {
-        float u = cell_coord(g, K1, i);
-        if (!(b.lo[K1] <= u && u <= b.hi[K1])) continue;
-        for (uint32_t j = c0; j <= c1 && c0 <= c1; ++j) {
-            float v = cell_coord(g, K2, j);
-            if (!(b.lo[K2] <= v && v <= b.hi[K2])) continue;
-            float o[3];
-            o[AXIS] = ox_along;
-            o[K1] = u;
-            o[K2] = v;
-            if (!(b.hi[AXIS] >= o[AXIS])) continue;  // box entirely behind the origin
-            float t;
-            if (!ray_hit<AXIS>(o[0], o[1], o[2], tr, &t)) continue;
-            // generate/grid.rs:605-607: (t / size).floor() as usize, capped at n - 1
-            float q = floorf(t / g.size[AXIS]);
-            uint32_t kcell;
-            if (!(q > 0.0f)) kcell = 0;  // NaN and negatives saturate to 0
-            else if (q >= (float)(g.n[AXIS] - 1)) kcell = g.n[AXIS] - 1;
-            else kcell = (uint32_t)q;
-            uint64_t row = (uint64_t)i * g.n[K2] + j;
-            atomicXor(&bits[row * g.words[AXIS] + (kcell >> 5)], 1u << (kcell & 31u));
-        }
+    RayRows r{0u, 0u, 0u, 0u};
+    if (!(b.hi[AXIS] >= cell_coord(g, AXIS, 0u))) return r;  // box entirely behind the origins
+    uint32_t a1, c1;
+    row_range(g, K1, b.lo[K1], b.hi[K1], r.a0, a1);
+    row_range(g, K2, b.lo[K2], b.hi[K2], r.c0, c1);
+    if (a1 >= r.a0 && c1 >= r.c0) {
+        r.na = a1 - r.a0 + 1;
+        r.nc = c1 - r.c0 + 1;
+    }
+    return r;
+}
+
+// One ray row (i, j) of triangle tr along AXIS: the exact per-row box test,
+// the aligned ray test and the parity flip of the hit's cell.
+template <int AXIS>
+__device__ __forceinline__ void ray_row(const TriRec& tr, const Box& b, const GridDev& g, uint32_t i, uint32_t j,
+                                        uint32_t* __restrict__ bits) {
+    constexpr int K1 = AXIS == 0 ? 1 : 0;
+    constexpr int K2 = AXIS == 2 ? 1 : 2;
+    const float u = cell_coord(g, K1, i), v = cell_coord(g, K2, j);
+    if (!(b.lo[K1] <= u && u <= b.hi[K1]) || !(b.lo[K2] <= v && v <= b.hi[K2])) return;
+    float o[3];
+    o[AXIS] = cell_coord(g, AXIS, 0u);
+    o[K1] = u;
+    o[K2] = v;
+    float t;
+    if (!ray_hit<AXIS>(o[0], o[1], o[2], tr, &t)) return;
+    // generate/grid.rs:605-607: (t / size).floor() as usize, capped at n - 1
+    const float q = floorf(t / g.size[AXIS]);
+    uint32_t kcell;
+    if (!(q > 0.0f)) kcell = 0;  // NaN and negatives saturate to 0
+    else if (q >= (float)(g.n[AXIS] - 1)) kcell = g.n[AXIS] - 1;
+    else kcell = (uint32_t)q;
+    const uint64_t row = (uint64_t)i * g.n[K2] + j;
+    atomicXor(&bits[row * g.words[AXIS] + (kcell >> 5)], 1u << (kcell & 31u));
+}
+
+// A (triangle, axis) with at most RAY_SMALL rows (a triangle spanning a few
+// cells has up to about 36: row_range adds a row on each side) walks them on
+// its own lane;
+// a larger one goes to a list whose entries each get a whole block, so a few
+// large triangles on a fine grid spread over many lanes instead of walking
+// all their rows on one.
+constexpr uint32_t RAY_SMALL = 64;
+
+struct RayBig {
+    uint64_t w;  // 3 t + axis
+    RayRows r;
+};
+
+template <int MODE>
+__device__ __forceinline__ void ray_rows_walk(const GridReq& R, uint64_t t, int axis, const RayRows& r, uint64_t k0,
+                                              uint64_t k1, uint64_t kstep, uint32_t* __restrict__ bx,
+                                              uint32_t* __restrict__ by, uint32_t* __restrict__ bz) {
+    const TriRec tr = expand_leaf(R.ltris[t - R.tri_off]);
+    Box b;
+    tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
+    for (uint64_t k = k0; k < k1; k += kstep) {
+        const uint32_t i = r.a0 + (uint32_t)(k / r.nc), j = r.c0 + (uint32_t)(k % r.nc);
+        if (axis == 0) ray_row<0>(tr, b, R.g, i, j, bx + R.row_off[0]);
+        else if (axis == 1) ray_row<1>(tr, b, R.g, i, j, by + R.row_off[1]);
+        else ray_row<2>(tr, b, R.g, i, j, bz + R.row_off[2]);
     }
 }
 
+template <int AXIS>
+__device__ __forceinline__ void ray_rows_small(const TriRec& tr, const Box& b, const GridDev& g, const RayRows& r,
+                                               uint32_t* __restrict__ bits) {
+    for (uint32_t i = 0; i < r.na; ++i)
+        for (uint32_t j = 0; j < r.nc; ++j) ray_row<AXIS>(tr, b, g, r.a0 + i, r.c0 + j, bits);
+}
+
+// One lane per (triangle, axis): walks a small box's rows, lists a large one.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_ray_scatter(uint64_t nt, ReqView<mode_batch<MODE>> reqs,
-                                                     const uint32_t* __restrict__ tri_req,
-                                                     uint32_t* __restrict__ bx, uint32_t* __restrict__ by,
-                                                     uint32_t* __restrict__ bz) {
-    uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= 3 * nt) return;
-    uint64_t t = id / 3;
-    int axis = (int)(id % 3);
-    const uint32_t q = mode_batch<MODE> ? tri_req[t] : 0u;
-    if (q == UINT32_MAX) return;  // a triangle of no request in this run
-    with_req<MODE>(reqs, q, [&](const GridReq& R, uint32_t) {
-        const TriRec tr = expand_leaf(R.ltris[t - R.tri_off]);
-        Box b;
-        tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
-        if (axis == 0) scatter_axis<0>(tr, b, R.g, bx + R.row_off[0]);
-        else if (axis == 1) scatter_axis<1>(tr, b, R.g, by + R.row_off[1]);
-        else scatter_axis<2>(tr, b, R.g, bz + R.row_off[2]);
-    });
+__global__ void __launch_bounds__(256) k_ray_small(uint64_t nt, ReqView<mode_batch<MODE>> reqs,
+                                                   const uint32_t* __restrict__ tri_req, RayBig* __restrict__ big,
+                                                   uint32_t* __restrict__ n_big, uint32_t* __restrict__ bx,
+                                                   uint32_t* __restrict__ by, uint32_t* __restrict__ bz) {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool listed = false;
+    RayRows r{0u, 0u, 0u, 0u};
+    if (id < 3 * nt) {
+        const uint64_t t = id / 3;
+        const int axis = (int)(id % 3);
+        const uint32_t q = mode_batch<MODE> ? tri_req[t] : 0u;
+        if (q != UINT32_MAX) {  // else a triangle of no request in this run
+            with_req<MODE>(reqs, q, [&](const GridReq& R, uint32_t) {
+                const TriRec tr = expand_leaf(R.ltris[t - R.tri_off]);
+                Box b;
+                tri_bbox(tr.ax, tr.ay, tr.az, tr.bx, tr.by, tr.bz, tr.cx, tr.cy, tr.cz, b);
+                r = axis == 0 ? ray_rows<0>(b, R.g) : (axis == 1 ? ray_rows<1>(b, R.g) : ray_rows<2>(b, R.g));
+                if ((uint64_t)r.na * r.nc > RAY_SMALL) {
+                    listed = true;
+                } else if (axis == 0) {
+                    ray_rows_small<0>(tr, b, R.g, r, bx + R.row_off[0]);
+                } else if (axis == 1) {
+                    ray_rows_small<1>(tr, b, R.g, r, by + R.row_off[1]);
+                } else {
+                    ray_rows_small<2>(tr, b, R.g, r, bz + R.row_off[2]);
+                }
+            });
+        }
+    }
+    // one atomic per wave appends its listed entries
+    const uint64_t m = __ballot(listed);
+    if (m) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(n_big, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (listed) big[base + __popcll(m & ((1ull << lane) - 1ull))] = RayBig{id, r};
+    }
+}
+
+// The listed (triangle, axis) entries, one block each at a time.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_ray_big(ReqView<mode_batch<MODE>> reqs, const uint32_t* __restrict__ tri_req,
+                                                 const RayBig* __restrict__ big, const uint32_t* __restrict__ n_big,
+                                                 uint32_t* __restrict__ bx, uint32_t* __restrict__ by,
+                                                 uint32_t* __restrict__ bz) {
+    const uint32_t count = *n_big;
+    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+        const RayBig E = big[e];
+        const uint64_t t = E.w / 3;
+        const int axis = (int)(E.w % 3);
+        const uint64_t nrows = (uint64_t)E.r.na * E.r.nc;
+        with_req<MODE>(reqs, mode_batch<MODE> ? tri_req[t] : 0u, [&](const GridReq& R, uint32_t) {
+            ray_rows_walk<MODE>(R, t, axis, E.r, threadIdx.x, nrows, blockDim.x, bx, by, bz);
+        });
+    }
 }
 
 // Per row: bit i becomes the XOR of flip bits i..n-1 (the parity of hits whose
@@ -1244,7 +1335,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     runs.end[1] = runs.end[0] + 2 * req_words;     // band minima: none yet
     runs.end[2] = runs.end[1] + req_words;         // taken counts of even rounds: 0
     runs.end[3] = runs.end[2] + req_words;         // taken counts read by round 0: large, so the band applies
-    runs.end[4] = runs.end[3] + 2 + nreq;          // totals, far-field flags: 0
+    runs.end[4] = runs.end[3] + 3 + nreq;          // totals, far-field flags, the ray stage's list count: 0
     runs.end[5] = runs.end[4] + bit_words;         // ray-parity bits: 0
     runs.value[0] = 0u;
     runs.value[1] = UINT32_MAX;
@@ -1347,8 +1438,11 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         bits[0] = words + runs.end[4];  // cleared by k_grid_init
         bits[1] = bits[0] + words_total[0];
         bits[2] = bits[1] + words_total[1];
-        SDFX_LAUNCH(k_ray_scatter<MODE>, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, nt, rv, tri_req, bits[0],
-                    bits[1], bits[2]);
+        RayBig* big = A.alloc<RayBig>(3 * nt);
+        uint32_t* n_big = totals + 2 + nreq;  // cleared by k_grid_init
+        SDFX_LAUNCH(k_ray_small<MODE>, dim3(blocks_for(3 * nt, 256)), dim3(256), 0, s, nt, rv, tri_req, big, n_big,
+                    bits[0], bits[1], bits[2]);
+        SDFX_LAUNCH(k_ray_big<MODE>, dim3(1024), dim3(256), 0, s, rv, tri_req, big, n_big, bits[0], bits[1], bits[2]);
         SDFX_LAUNCH(k_ray_suffix<MODE>, dim3(blocks_for(rows_total[0] + rows_total[1] + rows_total[2], 256)),
                     dim3(256), 0, s, bits[0], bits[1], bits[2], rows_total[0], rows_total[1], rows_total[2], rv,
                     nreq);
