@@ -282,6 +282,28 @@ def main():
                  "batched_over_plain": value / (samples / loop_s)}
     if args.workload == "serve" and not args.burst_only:
         forms = serve.forms(value, ms, step_ms)
+    if args.workload == "grid":
+        # what a drop-in caller pays: the host-pointer entry point, which also
+        # uploads the mesh and downloads the distances over PCIe (not `value`)
+        hv = np.ascontiguousarray(v, np.float32)
+        hf = np.ascontiguousarray(f.reshape(-1), np.uint32)
+        hout = np.empty(ncells, np.float32)
+        U32 = ctypes.POINTER(ctypes.c_uint32)
+
+        def host_call():
+            _lib.check(L.sdfx_generate_grid_sdf(
+                hv.ctypes.data_as(F), len(v), hf.ctypes.data_as(U32), hf.size, 0, first_a.ctypes.data_as(F),
+                size_a.ctypes.data_as(F), cnt_a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 0,
+                hout.ctypes.data_as(F)))
+
+        host_call()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            host_call()
+        host_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        forms = {"device_pointers": {"ms_per_call": ms, "samples_per_s": value},
+                 "host_pointers": {"ms_per_call": host_ms, "samples_per_s": samples / host_ms * 1e3,
+                                   "note": "sdfx_generate_grid_sdf: mesh upload and distance download included"}}
 
     # Kernel timing: the same K steps again with HIP events around every launch
     # (on the launch stream), plus exact work counters.
