@@ -316,7 +316,8 @@ def main():
     counters = {k: _lib.profile_counter(k) for k in (
         "grid.seed_distance_evals", "grid.propagate_distance_evals", "lbvh.nearest_nodes", "lbvh.nearest_leaves",
         "lbvh.vote_nodes", "lbvh.vote_leaves", "lbvh.queries", "lbvh.nearest_wave_slots",
-        "lbvh.vote_wave_slots", "lbvh.deferred_queries", "lbvh.deferred_nodes", "lbvh.deferred_leaves")}
+        "lbvh.vote_wave_slots", "lbvh.deferred_queries", "lbvh.deferred_nodes", "lbvh.deferred_leaves",
+        "lbvh.nearest_drops", "lbvh.restarts")}
     _lib.profile_enable(False)
     kernels = sorted(((name, t, n) for name, (t, n) in prof.items()), key=lambda x: -x[1])
     dom_name = kernels[0][0]
@@ -406,7 +407,8 @@ def main():
     if counters["lbvh.queries"]:
         nq_total = counters["lbvh.queries"]
         result["traversal_per_query"] = {k.split(".", 1)[1]: counters[k] / nq_total for k in (
-            "lbvh.nearest_nodes", "lbvh.nearest_leaves", "lbvh.vote_nodes", "lbvh.vote_leaves")}
+            "lbvh.nearest_nodes", "lbvh.nearest_leaves", "lbvh.vote_nodes", "lbvh.vote_leaves",
+            "lbvh.nearest_drops", "lbvh.restarts") if k in counters}
         if counters.get("lbvh.deferred_queries"):
             # nearest searches past the visit budget, finished one query per wave
             nd = counters["lbvh.deferred_queries"]

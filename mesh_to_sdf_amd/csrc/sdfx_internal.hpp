@@ -59,9 +59,9 @@ struct __attribute__((aligned(16))) BvhNode {
     float lmax[3];
     int32_t right;
     float rmin[3];
-    int32_t last;  // last leaf position the node covers
+    int32_t last;    // last leaf position the node covers
     float rmax[3];
-    uint32_t pad1;
+    int32_t parent;  // internal parent node, -1 at the root
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
 

@@ -386,9 +386,9 @@ __global__ void __launch_bounds__(256) k_refit(const BuildSeg* __restrict__ segs
         nd.left = l;
         nd.right = r;
         nd.last = last[p];
-        nd.pad1 = 0;
+        nd.parent = parent_int[p];
         nodes[p] = nd;
-        p = parent_int[p];
+        p = nd.parent;
     }
 }
 
@@ -500,61 +500,58 @@ struct RingStack {
     }
 };
 
-// Stack of the first pass's nearest search: the newest STACK_RING entries in
-// the LDS ring, older ones in a private array (scratch memory) until the ring
-// drains. Unlike the other traversals this one keeps every entry: half of the
-// uniform queries of a noise sphere need more than eight pending boxes within
-// their final bound, and recovering dropped ones (a restart under the final
-// bound, keeping either the newest or the nearest eight) cost more than the
-// spill (measured on the 327,680-triangle sphere: 35-39 ms per 4.19M queries
-// against 24.5 ms; the spill writes about 360 bytes per query, 1% of HBM
-// bandwidth over the pass). The spill array is its own
-// object: a private array indexed at run time lives in scratch memory, and a
-// struct holding one would take the counters along, so every push and pop
-// would write them there; kept apart, the counters stay in registers.
-constexpr int SPILL_DEPTH = 64;  // Karras depth <= 64 (32 code bits + 32 position bits)
+// Stack of the first pass's nearest search: a ring of NEAR_RING one-word
+// entries in the LDS words of the two-word ring. A pushed node is always
+// internal (leaf children are visited at once), so its index fits in the high
+// `nb` bits, where nb covers the tree's internal nodes, and the low 32 - nb
+// bits hold the top bits of its box's d^2: non-negative float bits order as
+// integers, so the dropped low bits make the stored value a lower bound, and
+// popping with it keeps every box the exact value keeps (the node's own
+// children are then tested with exact boxes).
+//
+// A push onto a full ring overwrites the oldest entry, which the ring hands
+// back; nearest_from keeps the newest dropped one and recovers the others by
+// climbing parent links (no scratch memory: half of the uniform queries of a
+// noise sphere hold more than eight pending boxes within their final bound).
+constexpr int NEAR_RING = 2 * STACK_RING;
 
-struct StackSpill {
-    uint32_t a[SPILL_DEPTH], b[SPILL_DEPTH];
+// The packing of one tree: `cap` bounds its node count (visit_cap, at least
+// twice the internal nodes), so internal indices stay below cap / 2.
+struct NearCode {
+    uint32_t dbits, dshift;  // d^2 bits kept and the shift that drops the rest
+    __device__ explicit NearCode(uint32_t cap) {
+        const uint32_t nb = 32u - (uint32_t)__clz((int)max(cap >> 1, 1u));  // 1..31
+        dbits = 32u - nb;
+        dshift = nb - 1u;  // 31 significant bits of a non-negative float, dbits kept
+    }
+    __device__ __forceinline__ uint32_t pack(int32_t node, float d2) const {
+        return ((uint32_t)node << dbits) | ((fbits(d2) & 0x7FFFFFFFu) >> dshift);
+    }
+    __device__ __forceinline__ int32_t node(uint32_t e) const { return (int32_t)(e >> dbits); }
+    __device__ __forceinline__ float d2_floor(uint32_t e) const {
+        return bitsf((e & ((1u << dbits) - 1u)) << dshift);
+    }
 };
 
-struct SpillStack {
-    uint32_t* col;  // this lane's ring column, as in RingStack
-    StackSpill& sp;
-    uint32_t& spills;  // statistics: entries moved to the spill array
-    int head = 0, count = 0, nspill = 0;
-    __device__ SpillStack(StackSpill& spill, uint32_t& n) : col(stack_lds() + threadIdx.x), sp(spill), spills(n) {}
-    __device__ __forceinline__ bool empty() const { return count == 0 && nspill == 0; }
-    // false when the stack is full (SPILL_DEPTH spilled entries): the entry is dropped
-    __device__ __forceinline__ bool push(uint32_t a, uint32_t b) {
-        if (count == STACK_RING) {  // move the oldest ring entry to the spill array
-            if (nspill == SPILL_DEPTH) return false;
-            const int old = head & (STACK_RING - 1);  // head - RING == head (mod RING)
-            sp.a[nspill] = col[2 * old * STACK_LANES];
-            sp.b[nspill] = col[(2 * old + 1) * STACK_LANES];
-            ++nspill;
-            --count;
-            ++spills;
-        }
-        const int h = head & (STACK_RING - 1);
-        col[2 * h * STACK_LANES] = a;
-        col[(2 * h + 1) * STACK_LANES] = b;
+struct NearRing {
+    uint32_t* col;  // this lane's ring column: entry k at col[k * STACK_LANES]
+    int head = 0, count = 0;
+    __device__ NearRing() : col(stack_lds() + threadIdx.x) {}
+    __device__ __forceinline__ bool empty() const { return count == 0; }
+    // Returns true when the ring was full: its oldest entry, now overwritten, is in `dropped`.
+    __device__ __forceinline__ bool push(uint32_t e, uint32_t& dropped) {
+        uint32_t* slot = col + (head & (NEAR_RING - 1)) * STACK_LANES;  // with a full ring, the oldest entry
+        const bool full = count == NEAR_RING;
+        if (full) dropped = *slot;
+        else ++count;
+        *slot = e;
         ++head;
-        ++count;
-        return true;
+        return full;
     }
-    __device__ __forceinline__ void pop(uint32_t& a, uint32_t& b) {  // requires !empty()
-        if (count > 0) {
-            --head;
-            --count;
-            const int h = head & (STACK_RING - 1);
-            a = col[2 * h * STACK_LANES];
-            b = col[(2 * h + 1) * STACK_LANES];
-        } else {
-            --nspill;
-            a = sp.a[nspill];
-            b = sp.b[nspill];
-        }
+    __device__ __forceinline__ uint32_t pop() {  // requires !empty()
+        --head;
+        --count;
+        return col[(head & (NEAR_RING - 1)) * STACK_LANES];
     }
 };
 
@@ -567,7 +564,7 @@ struct Nearest {
 // Traversal statistics gathered while kernel timing is on.
 struct TravStats {
     uint32_t nodes = 0, leaves = 0, vote_nodes = 0, vote_leaves = 0;
-    uint32_t spills = 0;     // nearest-search stack entries moved to scratch memory
+    uint32_t drops = 0;      // nearest-search ring entries dropped (recovered by climbing)
     uint32_t restarts = 0;   // restarts of exact traversals after the ring dropped an entry
 };
 
@@ -678,7 +675,7 @@ __device__ bool dfs_exact(const BvhNode* __restrict__ nodes, int32_t start, int3
 
 // Nearest search of one lane from the root, closest child first, folding
 // into the lane's running best. Returns false when more than `budget` nodes
-// were visited with nodes left (the search is incomplete; best holds what was
+// were entered with nodes left (the search is incomplete; best holds what was
 // found so far) if budget is below `cap`; more than `cap` visits means a
 // malformed tree (overflow).
 //
@@ -689,6 +686,20 @@ __device__ bool dfs_exact(const BvhNode* __restrict__ nodes, int32_t start, int3
 // the visiting order, start nodes, repeated visits or initial bound:
 // k_lbvh_coop, which splits one query's tree among the lanes of a wave,
 // returns the same answer as one lane.
+//
+// Pending boxes are the far children of the current path's ancestors, newest
+// (deepest) on top of the ring. A full ring drops its oldest entry, so the
+// dropped entries are the far children of the path's shallowest pushing
+// ancestors, and the newest dropped entry E is the deepest of them. Once the
+// ring drains, the search takes E up (if still within the bound) and, when
+// E's subtree is done, climbs E's parent links to the root: at each ancestor
+// it recomputes both child distances, which give the same near / far choice
+// as on the way down, and enters the far child if it came up from the near
+// one and the far box is still within the bound. Every ancestor above E's
+// parent pushed its far child before E (so it was dropped too) or never
+// pushed it (a leaf visited at once, or a box then outside the bound), so the
+// climb finds exactly the boxes the ring lost. A drop inside a subtree taken
+// up this way replaces E: its climb passes through that subtree's root.
 __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                              float px, float py, float pz, uint32_t budget, uint32_t cap, Nearest& best,
                              bool& overflow, TravStats& ts) {
@@ -697,15 +708,20 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* _
         leaf(~root);
         return true;
     }
-    StackSpill spill;
-    SpillStack st(spill, ts.spills);
+    NearRing st;
+    const NearCode code(cap);
     int32_t node = root;
     uint32_t visits = 0;
+    // the newest dropped entry not yet taken up, or NO_DROP (its node field
+    // 2^nb - 1 is no internal node's index: those stay below cap / 2)
+    constexpr uint32_t NO_DROP = 0xFFFFFFFFu;
+    uint32_t drop = NO_DROP;
+    int32_t resume = -1;  // once the ring drains, climb from this node (its subtree is then done)
     for (;;) {
         ++visits;
         if (visits > budget) {
             if (budget < cap) return false;
-            overflow = true;  // a tree visits each node once; more means a malformed tree
+            overflow = true;  // a tree enters each node once; more means a malformed tree
             return true;
         }
         ts.nodes++;
@@ -729,7 +745,11 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* _
         }
         if (gl && gr) {
             const bool left_first = dl <= dr;
-            if (!st.push((uint32_t)(left_first ? nd.right : nd.left), fbits(left_first ? dr : dl))) overflow = true;
+            uint32_t old;
+            if (st.push(code.pack(left_first ? nd.right : nd.left, left_first ? dr : dl), old)) {
+                drop = old;
+                ts.drops++;
+            }
             node = left_first ? nd.left : nd.right;
             continue;
         }
@@ -741,15 +761,49 @@ __device__ bool nearest_from(const BvhNode* __restrict__ nodes, const LeafRec* _
             node = nd.right;
             continue;
         }
+        // next: the newest ring entry still within the bound, else the newest
+        // dropped entry, else a far child found by climbing
         bool found = false;
         while (!st.empty()) {
-            uint32_t a, bb;
-            st.pop(a, bb);
-            if (bitsf(bb) * PRUNE_SLACK <= best.d2) {
-                node = (int32_t)a;
+            const uint32_t e = st.pop();
+            if (code.d2_floor(e) * PRUNE_SLACK <= best.d2) {
+                node = code.node(e);
                 found = true;
                 break;
             }
+        }
+        if (found) continue;
+        if (drop != NO_DROP) {
+            resume = code.node(drop);
+            const bool keep = code.d2_floor(drop) * PRUNE_SLACK <= best.d2;
+            drop = NO_DROP;
+            if (keep) {
+                node = resume;
+                continue;
+            }
+        }
+        if (resume < 0) break;
+        int32_t x = resume, p = nodes[resume].parent;
+        found = false;
+        for (int hops = 0; p >= 0; ++hops) {
+            if (hops >= 64) {
+                overflow = true;  // a Karras tree is at most 64 deep
+                return true;
+            }
+            ts.nodes++;
+            const BvhNode pn = nodes[p];
+            const float cl = box_dist2(px, py, pz, pn.lmin, pn.lmax);
+            const float cr = box_dist2(px, py, pz, pn.rmin, pn.rmax);
+            const bool left_first = cl <= cr;
+            const int32_t near = left_first ? pn.left : pn.right, far = left_first ? pn.right : pn.left;
+            if (x == near && far >= 0 && (left_first ? cr : cl) * PRUNE_SLACK <= best.d2) {
+                node = far;
+                resume = far;
+                found = true;
+                break;
+            }
+            x = p;
+            p = pn.parent;
         }
         if (!found) break;
     }
@@ -898,16 +952,18 @@ struct DeferRec {
 // query kernel below). With `defer`, a nearest search that passes `budget`
 // visits stops and fills *defer instead of out[i] (returns true); the vote
 // still runs here.
+// One query of the point paths at (px, py, pz). Returns true when the nearest
+// search ran out of its budget and the query goes to the per-wave pass (defer
+// holds its state; the caller sets defer.q); otherwise `ans` is the answer.
 template <int MODE>
 __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves,
-                                            int32_t root, uint32_t max_visits, const float* __restrict__ q,
-                                            uint64_t i, float* __restrict__ out, DevFlags* flags, TravStats& ts,
-                                            uint32_t budget, bool can_defer, DeferRec& defer) {
-    const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+                                            int32_t root, uint32_t max_visits, float px, float py, float pz,
+                                            DevFlags* flags, TravStats& ts, uint32_t budget, bool can_defer,
+                                            DeferRec& defer, float& ans) {
     bool overflow = false, deferred = false;
     if (MODE == QM_NORMAL_FOLD) {
         bool nan_seen = false;
-        out[i] = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow, ts);
+        ans = bvh_normal_fold(nodes, leaves, root, px, py, pz, max_visits, nan_seen, overflow, ts);
         if (nan_seen) atomicOr(&flags->nan_distance, 1u);
     } else {
         Nearest nb{INFINITY, 0xFFFFFFFFu, -1};
@@ -916,9 +972,9 @@ __device__ __forceinline__ bool query_point(const BvhNode* __restrict__ nodes, c
         const bool inside = MODE != QM_NEAREST_SIGNED &&
                             bvh_inside_vote(nodes, leaves, root, px, py, pz, max_visits, overflow, ts);
         if (done) {
-            out[i] = nearest_answer<MODE>(nb, inside, px, py, pz, leaves);
+            ans = nearest_answer<MODE>(nb, inside, px, py, pz, leaves);
         } else {
-            defer = DeferRec{i, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u, 0u, 0u};
+            defer = DeferRec{0ull, fbits(nb.d2), nb.index, nb.leaf, inside ? 1u : 0u, 0u, 0u};
             deferred = true;
         }
     }
@@ -942,8 +998,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     DeferRec rec;
     if (active) {
         if (order) i = order[i];  // lanes of a wave take Morton-adjacent queries
-        dfr = query_point<MODE>(nodes, leaves, root, max_visits, q, i, out, flags, ts, budget, deferred != nullptr,
-                                rec);
+        float ans;
+        dfr = query_point<MODE>(nodes, leaves, root, max_visits, q[3 * i], q[3 * i + 1], q[3 * i + 2], flags, ts,
+                                budget, deferred != nullptr, rec, ans);
+        if (dfr) rec.q = i;
+        else out[i] = ans;
     }
     if (deferred) {  // kernel argument: uniform; one atomic per wave appends its deferred queries
         const uint64_t m = __ballot(dfr);
@@ -971,7 +1030,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         }
         block_count(stats + 4 * nb, (threadIdx.x & 63u) == 0 ? 64u * mx : 0u);
         block_count(stats + 5 * nb, (threadIdx.x & 63u) == 0 ? 64u * mv : 0u);
-        block_count(stats + 6 * nb, ts.spills);
+        block_count(stats + 6 * nb, ts.drops);
         block_count(stats + 7 * nb, ts.restarts);
     }
 }
@@ -1283,7 +1342,7 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
         profile_count("lbvh.vote_leaves", sum_block_counters(stats + 3ull * blocks, blocks, s));
         profile_count("lbvh.nearest_wave_slots", sum_block_counters(stats + 4ull * blocks, blocks, s));
         profile_count("lbvh.vote_wave_slots", sum_block_counters(stats + 5ull * blocks, blocks, s));
-        profile_count("lbvh.nearest_spills", sum_block_counters(stats + 6ull * blocks, blocks, s));
+        profile_count("lbvh.nearest_drops", sum_block_counters(stats + 6ull * blocks, blocks, s));
         profile_count("lbvh.restarts", sum_block_counters(stats + 7ull * blocks, blocks, s));
         profile_count("lbvh.queries", nq);
         if (split) {
@@ -1352,9 +1411,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         const uint64_t g = order ? order[i] : i;
         const uint32_t r = point_req_of(reqs, nreq, g);
         TravStats ts;
+        // The query's index within its request waits in LDS across the
+        // traversal (all registers are taken there; 1 KB per block).
+        __shared__ uint32_t local_index[256];
         auto run = [&](const PointReq& R) {
-            dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q, g - R.q_off, R.out, flags + R.flag,
-                                    ts, R.budget, deferred != nullptr, rec);
+            const uint64_t k = g - R.q_off;  // below 2^32: a batch holds fewer than 2^32 queries
+            local_index[threadIdx.x] = (uint32_t)k;
+            float ans;
+            dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q[3 * k], R.q[3 * k + 1],
+                                    R.q[3 * k + 2], flags + R.flag, ts, R.budget, deferred != nullptr, rec, ans);
+            const uint32_t kk = local_index[threadIdx.x];
+            if (dfr) rec.q = kk;
+            else R.out[kk] = ans;
         };
         // queries lie request by request: a wave nearly always holds one, whose
         // fields (tree pointers among them) then sit in scalar registers
