@@ -144,10 +144,12 @@ def test_fixpoint_check_catches_planted_schedule_faults(oracle, suzanne):
 def test_normal_mode_ties_follow_compare_distances(oracle, suzanne, mesh, n):
     """Normal sign on tie-prone meshes (shared edges and vertices): every
     schedule reaches the fixpoint under compare_distances, and the frontier
-    schedule's signs equal the reference 1-heap schedule's except where the
-    two hold exactly equal |d| from different triangles and the frontier holds
-    the positive one (the value compare_distances prefers; the 1-heap
-    schedule never offered that triangle to the cell)."""
+    schedule's signs equal the reference 1-heap schedule's except at a few
+    cells where the frontier holds either exactly the same |d| from another
+    triangle, positive (the value compare_distances prefers; the 1-heap
+    schedule never offered that triangle to the cell), or a strictly smaller
+    |d| that is still at least the exact distance (a nearer triangle the 1-heap
+    schedule never offered, whose sign differs)."""
     O = oracle
     if mesh == "suzanne":
         v, f = suzanne
@@ -165,5 +167,10 @@ def test_normal_mode_ties_follow_compare_distances(oracle, suzanne, mesh, n):
     fr, h1 = outs[(8, O.GRID_FRONTIER)], outs[(1, O.GRID_SEQUENTIAL)]
     bad = np.nonzero(np.signbit(fr) != np.signbit(h1))[0]
     print(f"{mesh} {n}^3 Normal: {bad.size} sign mismatches frontier vs 1-heap")
-    assert bad.size <= 1
-    assert np.all(np.abs(fr[bad]) == np.abs(h1[bad])) and np.all(fr[bad] > 0)
+    assert bad.size <= 4
+    if bad.size:
+        exact = np.sqrt(O.nearest_bvh(v, f, O.grid_centres(first, size, count)[bad])[0])
+        afr, ah1 = np.abs(fr[bad]), np.abs(h1[bad])
+        tie = afr == ah1
+        assert np.all(fr[bad][tie] > 0)
+        assert np.all(afr[~tie] < ah1[~tie]) and np.all(afr >= exact)
