@@ -301,9 +301,30 @@ def main():
         for _ in range(args.steps):
             host_call()
         host_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        # the schedule-free grid on the same device buffers: each cell's exact
+        # FP32 nearest distance (no counterpart in the reference)
+        dexact = torch.empty_like(dout)
+
+        def exact_call():
+            _lib.check(L.sdfx_generate_grid_sdf_exact_device(
+                ctypes.c_void_p(dv.data_ptr()), len(v), ctypes.c_void_p(df.data_ptr()), df.numel(), 0,
+                first_a.ctypes.data_as(F), size_a.ctypes.data_as(F),
+                cnt_a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 0, ctypes.c_void_p(dexact.data_ptr()), sp))
+
+        exact_call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            exact_call()
+        torch.cuda.synchronize()
+        exact_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         forms = {"device_pointers": {"ms_per_call": ms, "samples_per_s": value},
                  "host_pointers": {"ms_per_call": host_ms, "samples_per_s": samples / host_ms * 1e3,
-                                   "note": "sdfx_generate_grid_sdf: mesh upload and distance download included"}}
+                                   "note": "sdfx_generate_grid_sdf: mesh upload and distance download included"},
+                 "exact": {"ms_per_call": exact_ms, "samples_per_s": samples / exact_ms * 1e3,
+                           "cells_differing_from_flood_fill": int((dexact != dout).sum().item()),
+                           "note": "sdfx_generate_grid_sdf_exact_device: every cell the FP32 nearest distance, "
+                                   "signs from the same grid raycasts"}}
 
     # Kernel timing: the same K steps again with HIP events around every launch
     # (on the launch stream), plus exact work counters.

@@ -125,6 +125,27 @@ sdfx_status sdfx_generate_grid_sdf_device(const float* d_vertices, uint64_t vert
                                           const uint64_t cell_count[3], sdfx_sign_method sign, float* d_out,
                                           void* stream);
 
+/* No counterpart in the reference: a schedule-free generate_grid_sdf for
+ * callers that need each cell's distance exact rather than the flood fill's
+ * approximation, whose error in the reference depends on its thread count.
+ * Same arguments, errors and sign as sdfx_generate_grid_sdf; each cell's
+ * magnitude is the FP32 minimum over all triangles of geo.rs's distance from
+ * the cell centre (Raycast, signs from the same grid raycasts as
+ * generate/grid.rs:568-684), or the compare_distances fold over all triangles
+ * of geo.rs's signed distance (Normal, as Bvh(Normal) answers a query point,
+ * generic/bvh.rs:117-129). */
+sdfx_status sdfx_generate_grid_sdf_exact(const float* vertices, uint64_t vertex_count, const uint32_t* indices,
+                                         uint64_t index_count, sdfx_topology topology, const float first_cell[3],
+                                         const float cell_size[3], const uint64_t cell_count[3],
+                                         sdfx_sign_method sign, float* out);
+
+/* Device-pointer form of sdfx_generate_grid_sdf_exact. */
+sdfx_status sdfx_generate_grid_sdf_exact_device(const float* d_vertices, uint64_t vertex_count,
+                                                const uint32_t* d_indices, uint64_t index_count,
+                                                sdfx_topology topology, const float first_cell[3],
+                                                const float cell_size[3], const uint64_t cell_count[3],
+                                                sdfx_sign_method sign, float* d_out, void* stream);
+
 /* One request of a batched generate_grid_sdf (the fields of
  * sdfx_generate_grid_sdf's arguments); `status` is set by the call. */
 typedef struct sdfx_grid_request {

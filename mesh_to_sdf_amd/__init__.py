@@ -31,7 +31,7 @@ __all__ = [
     "SignMethod", "AccelerationMethod", "ALL_SDF_ACCELERATION_METHODS", "Topology", "Grid", "SnapResult",
     "generate_sdf", "generate_grid_sdf", "build_sdf_acceleration", "query_sdf",
     "query_vector_to_closest_point_rtree_bvh", "SdfAccelerationMesh", "SdfxError", "check_mesh_triangle_list",
-    "generate_grid_sdf_batch", "query_sdf_batch",
+    "generate_grid_sdf_batch", "query_sdf_batch", "generate_grid_sdf_exact",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -239,18 +239,32 @@ def generate_sdf(vertices, indices: Topology, query_points, acceleration_method:
     return out[: n.value]
 
 
-def generate_grid_sdf(vertices, indices: Topology, grid: Grid, sign_method: int = SignMethod.Raycast) -> np.ndarray:
-    """generate/grid.rs:265-273. Returns float32 distances, z fastest."""
+def _grid_call(fn, vertices, indices: Topology, grid: Grid, sign_method: int) -> np.ndarray:
     v = _verts(vertices)
     cnt = np.array(grid.cell_count, dtype=np.uint64)
     out = np.empty(grid.get_total_cell_count(), dtype=np.float32)
     idx = indices.indices
     first = np.ascontiguousarray(grid.first_cell, dtype=np.float32)
     size = np.ascontiguousarray(grid.cell_size, dtype=np.float32)
-    _lib.check(_lib.lib().sdfx_generate_grid_sdf(
-        _ptr(v, _F), len(v), _ptr(idx, _U32), 0 if idx is None else idx.size, indices.kind, _ptr(first, _F),
-        _ptr(size, _F), _ptr(cnt, _U64), sign_method, _ptr(out, _F)))
+    _lib.check(fn(_ptr(v, _F), len(v), _ptr(idx, _U32), 0 if idx is None else idx.size, indices.kind,
+                  _ptr(first, _F), _ptr(size, _F), _ptr(cnt, _U64), sign_method, _ptr(out, _F)))
     return out
+
+
+def generate_grid_sdf(vertices, indices: Topology, grid: Grid, sign_method: int = SignMethod.Raycast) -> np.ndarray:
+    """generate/grid.rs:265-273. Returns float32 distances, z fastest."""
+    return _grid_call(_lib.lib().sdfx_generate_grid_sdf, vertices, indices, grid, sign_method)
+
+
+def generate_grid_sdf_exact(vertices, indices: Topology, grid: Grid,
+                            sign_method: int = SignMethod.Raycast) -> np.ndarray:
+    """generate_grid_sdf without the flood fill's approximation (no counterpart
+    in the reference): each cell holds the FP32 minimum over all triangles of
+    the distance from its centre (Raycast, signed by the same grid raycasts as
+    generate/grid.rs:568-684) or the compare_distances fold of the signed
+    distances (Normal, as Bvh(Normal) answers a point, generic/bvh.rs:117-129).
+    Same arguments and errors as generate_grid_sdf."""
+    return _grid_call(_lib.lib().sdfx_generate_grid_sdf_exact, vertices, indices, grid, sign_method)
 
 
 def generate_grid_sdf_batch(requests, sign_method: int = SignMethod.Raycast, return_status: bool = False):

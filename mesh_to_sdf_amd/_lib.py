@@ -72,6 +72,8 @@ SIGNATURES = {
     "sdfx_generate_sdf_device": (_i, [_vp, _u64v, _vp, _u64v, _i, _vp, _u64v, _i, _i, _vp, _u64, _vp]),
     "sdfx_generate_grid_sdf": (_i, [_f, _u64v, _u32, _u64v, _i, _f, _f, _u64, _i, _f]),
     "sdfx_generate_grid_sdf_device": (_i, [_vp, _u64v, _vp, _u64v, _i, _f, _f, _u64, _i, _vp, _vp]),
+    "sdfx_generate_grid_sdf_exact": (_i, [_f, _u64v, _u32, _u64v, _i, _f, _f, _u64, _i, _f]),
+    "sdfx_generate_grid_sdf_exact_device": (_i, [_vp, _u64v, _vp, _u64v, _i, _f, _f, _u64, _i, _vp, _vp]),
     "sdfx_build_sdf_acceleration": (_i, [_f, _u64v, _u32, _u64v, _i, _i, _i, ctypes.POINTER(_vp)]),
     "sdfx_query_sdf": (_i, [_vp, _f, _u64v, _f, _u64]),
     "sdfx_query_sdf_noalloc": (_i, [_vp, _vp, _u64v, _vp, _u64, _vp]),

@@ -176,7 +176,7 @@ sdfx_status generate_sdf_core(const float* d_v, uint64_t nv, const uint32_t* d_i
 
 sdfx_status grid_core(const float* d_v, uint64_t nv, const uint32_t* d_idx, uint64_t ni, sdfx_topology topo,
                       const float* first, const float* size, const uint64_t* count, sdfx_sign_method sign,
-                      float* d_out, hipStream_t s) {
+                      float* d_out, hipStream_t s, bool exact = false) {
     check_enums(topo, SDFX_ACCEL_NONE, sign);
     if (!first || !size || !count) fail(SDFX_ERR_INVALID_ARGUMENT, "null grid description");
     uint64_t nt = 0;
@@ -204,7 +204,10 @@ sdfx_status grid_core(const float* d_v, uint64_t nv, const uint32_t* d_idx, uint
     Arena A(s);
     DevFlags* flags = new_flags(A);
     gather_triangles(d_v, nv, d_idx, topo, nt, tris.get(), flags, s);
-    if (!(zero_axis && nt > 0)) grid_sdf(tris.get(), nt, g, sign, d_out, flags, A);
+    if (!(zero_axis && nt > 0)) {
+        if (exact) grid_sdf_exact(tris.get(), nt, g, sign, d_out, flags, A);
+        else grid_sdf(tris.get(), nt, g, sign, d_out, flags, A);
+    }
     DevFlags f = read_flags(flags, s);
     if (f.index_out_of_range) fail_index(nv);
     check_internal(f);
@@ -421,6 +424,43 @@ sdfx_status sdfx_generate_grid_sdf_device(const float* d_vertices, uint64_t vert
         require(cell_count != nullptr, "cell_count");
         return grid_core(d_vertices, vertex_count, d_indices, index_count, topology, first_cell, cell_size,
                          cell_count, sign, d_out, (hipStream_t)stream);
+    });
+}
+
+sdfx_status sdfx_generate_grid_sdf_exact(const float* vertices, uint64_t vertex_count, const uint32_t* indices,
+                                         uint64_t index_count, sdfx_topology topology, const float first_cell[3],
+                                         const float cell_size[3], const uint64_t cell_count[3],
+                                         sdfx_sign_method sign, float* out) {
+    return guarded([&]() -> sdfx_status {
+        require(vertices || vertex_count == 0, "vertices");
+        require(cell_count != nullptr, "cell_count");
+        uint64_t ncells = cell_count[0] * cell_count[1] * cell_count[2];
+        require(out || ncells == 0, "out");
+        hipStream_t s = thread_stream();
+        DevBuf<float> dv, dout(ncells);
+        DevBuf<uint32_t> di;
+        upload(dv, vertices, 3 * vertex_count, s);
+        if (indices) upload(di, indices, index_count, s);
+        sdfx_status st = grid_core(dv.get(), vertex_count, indices ? di.get() : nullptr, index_count, topology,
+                                   first_cell, cell_size, cell_count, sign, dout.get(), s, true);
+        if (st == SDFX_OK && ncells) {
+            SDFX_HIP(hipMemcpyAsync(out, dout.get(), ncells * sizeof(float), hipMemcpyDeviceToHost, s));
+            SDFX_HIP(hipStreamSynchronize(s));
+        }
+        return st;
+    });
+}
+
+sdfx_status sdfx_generate_grid_sdf_exact_device(const float* d_vertices, uint64_t vertex_count,
+                                                const uint32_t* d_indices, uint64_t index_count,
+                                                sdfx_topology topology, const float first_cell[3],
+                                                const float cell_size[3], const uint64_t cell_count[3],
+                                                sdfx_sign_method sign, float* d_out, void* stream) {
+    return guarded([&]() -> sdfx_status {
+        require(d_vertices || vertex_count == 0, "vertices");
+        require(cell_count != nullptr, "cell_count");
+        return grid_core(d_vertices, vertex_count, d_indices, index_count, topology, first_cell, cell_size,
+                         cell_count, sign, d_out, (hipStream_t)stream, true);
     });
 }
 

@@ -477,6 +477,35 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
     if (evals) block_count(evals, n_evals);
 }
 
+// ---------------------------------------------------------------- exact values
+
+// The cell centres of one grid as query points, cell i = (x * ny + y) * nz + z.
+__global__ void k_cell_centres(GridDev g, uint64_t ncells, float* __restrict__ q) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t z = (uint32_t)(i % g.n[2]);
+        const uint64_t r = i / g.n[2];
+        q[3 * i] = cell_coord(g, 0, (uint32_t)(r / g.n[1]));
+        q[3 * i + 1] = cell_coord(g, 1, (uint32_t)(r % g.n[1]));
+        q[3 * i + 2] = cell_coord(g, 2, z);
+    }
+}
+
+// Keys of a single request from per-cell values (the triangle field is not
+// read by the finish).
+template <int MODE>
+__global__ void k_keys_from_values(const float* __restrict__ values, uint64_t ncells, uint64_t* __restrict__ keys,
+                                   DevFlags* flags) {
+    constexpr int SIGN = mode_sign<MODE>;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float d = values[i];
+        if (SIGN == SDFX_SIGN_RAYCAST) d = fabsf(d);
+        if (d != d) atomicOr(&flags->nan_distance, 1u);
+        keys[i] = key_valid(d) ? make_key<SIGN>(d, 0u) : KEY_EMPTY;
+    }
+}
+
 // ---------------------------------------------------------------- propagation
 
 // Flags every seeded cell (round 0 takes the seeds) and folds each request's
@@ -1353,7 +1382,10 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     uint64_t* counts = A.alloc<uint64_t>(nt);
     uint64_t* total = A.alloc<uint64_t>(1);
     SDFX_LAUNCH(k_seed_boxes<MODE>, dim3(blocks_for(nt, 256)), dim3(256), 0, s, nt, rv, tri_req, boxes, counts);
-    exclusive_scan_u64(counts, counts, nt, total, A);
+    // a schedule-free request brings its values: its keys come from them and
+    // the seeding pairs and frontier rounds are skipped (the 48-byte triangle
+    // copies k_seed_boxes wrote serve the ray stage)
+    const bool exact = nreq == 1 && jobs[0].exact != nullptr;
     // a batch's rounds are large: one resident generation of blocks does best there
     const int64_t blocks_knob = mode_batch<MODE> ? tuning().grid_batch_prop_blocks : tuning().grid_prop_blocks;
     const unsigned prop_blocks = (unsigned)std::max<int64_t>(XCDS, blocks_knob) / XCDS * XCDS;
@@ -1361,10 +1393,16 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // per block of k_seed_pairs (8192 blocks), then one per propagation block
     unsigned long long* evals = nullptr;
     const uint64_t seed_slots = 8192, prop_slots = prop_blocks;
-    if (profile_enabled()) {
+    if (profile_enabled() && !exact) {
         evals = A.alloc<unsigned long long>(seed_slots + prop_slots);
         fill_u32(reinterpret_cast<uint32_t*>(evals), 2 * (seed_slots + prop_slots), 0u, s);
     }
+    uint32_t* totals = words + 2 * cls_words + 4 * req_words;
+    if (exact) {
+        SDFX_LAUNCH(k_keys_from_values<MODE>, dim3(stride_blocks(reqs[0].ncells)), dim3(256), 0, s, jobs[0].exact,
+                    (uint64_t)reqs[0].ncells, keys, d_flags + reqs[0].flag);
+    } else {
+    exclusive_scan_u64(counts, counts, nt, total, A);
     SDFX_LAUNCH(k_seed_pairs<MODE>, dim3(8192), dim3(256), 0, s, nt, rv, tri_req, boxes, counts, total, keys,
                 d_flags, evals);
     SDFX_HIP(hipGetLastError());
@@ -1373,7 +1411,6 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     uint32_t* ctr[2] = {words, words + cls_words};
     uint32_t* minw[2] = {words + 2 * cls_words, words + 2 * cls_words + req_words};
     uint32_t* taken[2] = {words + 2 * cls_words + 2 * req_words, words + 2 * cls_words + 3 * req_words};
-    uint32_t* totals = words + 2 * cls_words + 4 * req_words;
     uint32_t* released = totals + 2;  // per request: the far-field release happened
     const uint32_t region = (uint32_t)((ntiles + NCLASS - 1) / NCLASS * COMPACT_TILE);  // items per class
     uint8_t* changed = A.alloc<uint8_t>(ncells);  // whole tiles: 16-byte vector reads stay inside
@@ -1431,6 +1468,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         if (taken_cells == 0) break;  // the batch after it only runs empty rounds
         batch = taken_cells < 512 ? 2 : (taken_cells < 8192 ? 4 : 8);
     }
+    }  // frontier rounds
 
     // 3. sign
     uint32_t* bits[3] = {nullptr, nullptr, nullptr};
@@ -1519,6 +1557,35 @@ void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method si
 void grid_sdf(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
               DevFlags* d_flags, Arena& A) {
     grid_sdf_batch({GridBatchItem{d_tris, 0, nt, gp, d_out, 0}}, sign, d_flags, A, nullptr);
+}
+
+void grid_sdf_exact(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
+                    DevFlags* d_flags, Arena& A) {
+    const uint64_t ncells = gp.count[0] * gp.count[1] * gp.count[2];
+    if (!ncells || !nt) {  // the same empty-grid and empty-mesh conventions
+        grid_sdf(d_tris, nt, gp, sign, d_out, d_flags, A);
+        return;
+    }
+    Arena R(A.stream);  // the tree, the queries and the values, returned after the run
+    Lbvh bvh;
+    lbvh_build(d_tris, nt, bvh, R);
+    GridDev g{};
+    for (int k = 0; k < 3; ++k) {
+        g.first[k] = gp.first[k];
+        g.size[k] = gp.size[k];
+        g.n[k] = (uint32_t)gp.count[k];
+    }
+    float* q = R.alloc<float>(3 * ncells);
+    SDFX_LAUNCH(k_cell_centres, dim3(stride_blocks(ncells)), dim3(256), 0, R.stream, g, ncells, q);
+    float* values = R.alloc<float>(ncells);
+    // Raycast: nearest by (d^2, topology index), signed distance to it (its
+    // magnitude is the reference's point_triangle_distance to the nearest
+    // triangle); Normal: the compare_distances fold over every triangle
+    lbvh_query(bvh, q, ncells, sign == SDFX_SIGN_NORMAL ? QM_NORMAL_FOLD : QM_NEAREST_SIGNED, values, d_flags, R);
+    GridBatchItem it{d_tris, 0, nt, gp, d_out, 0};
+    it.exact = values;
+    if (sign == SDFX_SIGN_NORMAL) grid_run<SDFX_SIGN_NORMAL>(nt, {it}, d_flags, R);
+    else grid_run<SDFX_SIGN_RAYCAST>(nt, {it}, d_flags, R);
 }
 
 }  // namespace sdfx

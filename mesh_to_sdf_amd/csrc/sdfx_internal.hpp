@@ -176,12 +176,24 @@ struct GridBatchItem {
     GridParams gp;
     float* d_out;
     uint32_t flag;
+    // grid_sdf_exact: each cell's distance from the point queries (|d| for
+    // Raycast, the signed value for Normal), which stands in for seeding and
+    // propagation; the ray stage and the output are as for any request
+    const float* exact = nullptr;
 };
 // All requests (one sign method) through shared launches, in runs of at most
 // about 2^28 cells each; each result is bit-identical to grid_sdf on that
 // request alone. With run_errors, a run that fails records its error for its
 // requests (run_errors[i] for items[i]) and the others still run; without,
 // the error is thrown.
+// The schedule-free grid: each cell's distance is the FP32 minimum over all
+// triangles, found by the LBVH nearest search at the cell centre (Normal: the
+// compare_distances fold of Bvh(Normal)); Raycast signs come from the same
+// grid raycasts as grid_sdf's. No counterpart in the reference, whose flood
+// fill only approaches these distances, by an amount that depends on its
+// thread count.
+void grid_sdf_exact(const TriRec* d_tris, uint64_t nt, const GridParams& gp, sdfx_sign_method sign, float* d_out,
+                    DevFlags* d_flags, Arena& arena);
 void grid_sdf_batch(const std::vector<GridBatchItem>& items, sdfx_sign_method sign, DevFlags* d_flags,
                     Arena& arena, std::vector<std::pair<sdfx_status, std::string>>* run_errors);
 

@@ -29,7 +29,7 @@ ERR_INVALID_GRID = -10
 TOPO_LIST, TOPO_STRIP = 0, 1
 SIGN_RAYCAST, SIGN_NORMAL = 0, 1
 ACCEL_NONE, ACCEL_BVH, ACCEL_RTREE, ACCEL_RTREE_BVH = 0, 1, 2, 3
-GRID_SEQUENTIAL, GRID_THREADED, GRID_FRONTIER = 0, 1, 2
+GRID_SEQUENTIAL, GRID_THREADED, GRID_FRONTIER, GRID_EXACT = 0, 1, 2, 3
 FAULT_NONE, FAULT_SKIP_DIRECTION, FAULT_DROP_DEFERRED, FAULT_PACKED_ORDER = 0, 1, 2, 3
 NO_TRIANGLE = 0xFFFFFFFF
 

@@ -1271,6 +1271,46 @@ int orc_generate_grid_sdf_ex(const float* verts, uint64_t nv, const uint32_t* id
         free(tris);
         return ORC_ERR_INVALID_GRID;
     }
+    if (schedule == ORC_GRID_EXACT) {
+        float* dist_arr = (float*)malloc(sizeof(float) * (ncells + 1));
+        int st2 = ORC_OK;
+        for (uint64_t i = 0; i < ncells; ++i) {
+            dist_arr[i] = FLT_MAX; /* grid.rs:137-143, kept without triangles */
+            if (tri_out) tri_out[i] = UINT32_MAX;
+        }
+        if (nt > 0 && ncells > 0 && sign == ORC_SIGN_RAYCAST) {
+            obvh b;
+            bvh_build(&b, verts, tris, nt);
+            int nan_hit = 0;
+#pragma omp parallel for num_threads(n_heaps) schedule(dynamic, 256) reduction(| : nan_hit)
+            for (int64_t i = 0; i < (int64_t)ncells; ++i) {
+                uint64_t z = (uint64_t)i % g.n[2], y = ((uint64_t)i / g.n[2]) % g.n[1], x = (uint64_t)i / (g.n[1] * g.n[2]);
+                float bd;
+                uint32_t bt;
+                bvh_nearest(&b, verts, tris, cell_center(&g, x, y, z), &bd, &bt);
+                if (bd != bd) nan_hit = 1;
+                dist_arr[i] = sqrtf(bd); /* point.rs:114-121 length of the nearest difference */
+                if (tri_out) tri_out[i] = bt;
+            }
+            bvh_free(&b);
+            if (nan_hit) st2 = ORC_ERR_NAN_DISTANCE;
+            else grid_raycast_sign(&g, verts, tris, nt, dist_arr, n_heaps, &lst);
+        } else if (nt > 0 && ncells > 0) {
+            float* centres = (float*)malloc(sizeof(float) * 3 * ncells);
+            for (uint64_t i = 0; i < ncells; ++i) {
+                uint64_t z = i % g.n[2], y = (i / g.n[2]) % g.n[1], x = i / (g.n[1] * g.n[2]);
+                st(centres + 3 * i, cell_center(&g, x, y, z));
+            }
+            st2 = orc_generate_sdf(verts, nv, idx, ni, topo, centres, ncells, ORC_ACCEL_BVH, ORC_SIGN_NORMAL, dist_arr,
+                                   n_heaps, NULL);
+            free(centres);
+        }
+        if (st2 == ORC_OK) memcpy(out, dist_arr, sizeof(float) * ncells);
+        free(dist_arr);
+        free(tris);
+        if (stats) *stats = lst;
+        return st2;
+    }
     if (schedule == ORC_GRID_FRONTIER) {
         float* dist_arr = (float*)malloc(sizeof(float) * (ncells + 1));
         uint64_t evals = 0;

@@ -39,7 +39,7 @@ enum { ORC_TOPO_LIST = 0, ORC_TOPO_STRIP = 1 };
 enum { ORC_SIGN_RAYCAST = 0, ORC_SIGN_NORMAL = 1 };
 enum { ORC_ACCEL_NONE = 0, ORC_ACCEL_BVH = 1, ORC_ACCEL_RTREE = 2, ORC_ACCEL_RTREE_BVH = 3 };
 /* grid propagation schedules */
-enum { ORC_GRID_SEQUENTIAL = 0, ORC_GRID_THREADED = 1, ORC_GRID_FRONTIER = 2 };
+enum { ORC_GRID_SEQUENTIAL = 0, ORC_GRID_THREADED = 1, ORC_GRID_FRONTIER = 2, ORC_GRID_EXACT = 3 };
 
 /* ---- geometry (geo.rs) ---- */
 void orc_triangle_bounding_box(const float* a, const float* b, const float* c, float* lo, float* hi);
@@ -111,7 +111,16 @@ typedef struct {
  * flagged cell: in the far field the front carries mostly final values, so the
  * band there only adds rounds. It is deterministic and is the schedule the
  * device pipeline runs, so it pins the device kernels bit-exactly; its
- * distance from the reference's own schedules is measured separately. */
+ * distance from the reference's own schedules is measured separately.
+ * schedule ORC_GRID_EXACT: not one of the reference's schedules either: the
+ * value the flood fill approaches, with no schedule. Raycast: each cell holds
+ * the FP32 minimum over all triangles of geo.rs's distance from its centre
+ * (the triangle of the smallest d^2, the lowest index among equal ones),
+ * signed by the same grid raycasts (generate/grid.rs:568-684); Normal: the
+ * compare_distances fold of geo.rs's signed distances over all triangles, the
+ * value Bvh(Normal) gives the cell centre as a query point
+ * (generic/bvh.rs:117-129). tri_out receives the nearest triangle (Raycast)
+ * or UINT32_MAX (Normal). */
 #define ORC_GRID_BAND_CELLS 3.0f
 #define ORC_GRID_RELEASE_MIN 1024u
 #define ORC_GRID_RELEASE_DIV 128u
