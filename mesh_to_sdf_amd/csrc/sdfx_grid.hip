@@ -195,6 +195,81 @@ __device__ __forceinline__ float cell_dist(const GridDev& g, uint32_t x, uint32_
     return SIGN == SDFX_SIGN_RAYCAST ? tri_dist(px, py, pz, t) : tri_signed_dist(px, py, pz, t);
 }
 
+// cell_dist, also returning the vector from the closest point to the cell
+// centre (the same FP32 operations as tri_dist / tri_signed_dist).
+template <int SIGN>
+__device__ __forceinline__ float cell_dist_dir(const GridDev& g, uint32_t x, uint32_t y, uint32_t z, const TriRec& t,
+                                               float& dx, float& dy, float& dz) {
+    const float px = cell_coord(g, 0, x), py = cell_coord(g, 1, y), pz = cell_coord(g, 2, z);
+    float cx, cy, cz;
+    closest_diff_flat(px, py, pz, t, dx, dy, dz, cx, cy, cz);
+    const float d = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+    if (SIGN == SDFX_SIGN_RAYCAST) return d;
+    return dot3(dx, dy, dz, t.nx, t.ny, t.nz) > 0.0f ? d : -d;
+}
+
+// ------------------------------------------------ supporting-plane skip
+
+// Direction word of a cell: the vector from its closest point to its centre as
+// its last improvement computed it, over its largest component, in three
+// signed 10-bit fields (0 when the vector is zero or not finite). It only
+// steers how much the bound below skips: any value gives a valid bound, so the
+// words need no clearing and racing improvements may leave any of theirs.
+__device__ __forceinline__ uint32_t dir_pack(float dx, float dy, float dz) {
+    const float m = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    if (!(m > 0.0f) || !(m <= 3.4e38f)) return 0u;
+    const float sc = 511.0f / m;
+    const int qx = (int)rintf(dx * sc), qy = (int)rintf(dy * sc), qz = (int)rintf(dz * sc);
+    return ((uint32_t)qx & 1023u) | (((uint32_t)qy & 1023u) << 10) | (((uint32_t)qz & 1023u) << 20);
+}
+
+// Lower bound of a triangle's distance around an item's cell. For any unit
+// vector u the triangle T lies in the half-space u.x <= h, h = max over its
+// vertices of u.v, so every point n has d(n, T) >= u.n - h: with n = c + o
+// (c the item's cell centre), d(n, T) >= g + u.o, g = u.c - h. g is lowered by
+// a margin that covers the FP32 rounding of g, of u.o and of the reference's
+// distance, whose closest point lies within a few ulps of T: 2^-16 of the
+// largest coordinate involved plus 2^-12 of T's longest edge. A sliver
+// (|ab x ac|^2 below 2^-8 |ab|^2 |ac|^2), whose closest point can stray
+// further, gets no bound (g = -inf), nor does a zero direction word. A pair
+// whose bound exceeds the neighbour's key distance cannot improve it, so
+// skipping it changes no result. tests/test_grid_fixpoint.py checks the bound
+// on the CPU, with exact and with random directions, on meshes, degenerate
+// soups and slivers far from the origin.
+struct PlaneBound {
+    float ux, uy, uz, g;
+};
+__device__ __forceinline__ PlaneBound plane_bound(uint32_t w, float cx, float cy, float cz, const TriRec& t,
+                                                  float max_size) {
+    PlaneBound b{0.0f, 0.0f, 0.0f, -INFINITY};
+    float ux = (float)(((int)(w << 22)) >> 22), uy = (float)(((int)(w << 12)) >> 22),
+          uz = (float)(((int)(w << 2)) >> 22);
+    const float l2 = dot3(ux, uy, uz, ux, uy, uz);
+    const float ab2 = dot3(t.abx, t.aby, t.abz, t.abx, t.aby, t.abz);
+    const float ac2 = dot3(t.acx, t.acy, t.acz, t.acx, t.acy, t.acz);
+    const float bc2 = dot3(t.bcx, t.bcy, t.bcz, t.bcx, t.bcy, t.bcz);
+    const float n2 = dot3(t.nx, t.ny, t.nz, t.nx, t.ny, t.nz);
+    if (!(l2 > 0.0f) || (t.degen == 0u && !(n2 >= (ab2 * ac2) * (1.0f / 256.0f)))) return b;
+    const float inv = 1.0f / sqrtf(l2);
+    ux *= inv;
+    uy *= inv;
+    uz *= inv;
+    const float h = fmaxf(dot3(ux, uy, uz, t.ax, t.ay, t.az),
+                          fmaxf(dot3(ux, uy, uz, t.bx, t.by, t.bz), dot3(ux, uy, uz, t.cx, t.cy, t.cz)));
+    float sc = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
+    sc = fmaxf(sc, fmaxf(fmaxf(fabsf(t.ax), fabsf(t.ay)), fabsf(t.az)));
+    sc = fmaxf(sc, fmaxf(fmaxf(fabsf(t.bx), fabsf(t.by)), fabsf(t.bz)));
+    sc = fmaxf(sc, fmaxf(fmaxf(fabsf(t.cx), fabsf(t.cy)), fabsf(t.cz)));
+    sc = sc + max_size;
+    const float e = sqrtf(fmaxf(ab2, fmaxf(ac2, bc2)));
+    const float g = (dot3(ux, uy, uz, cx, cy, cz) - h) - (sc * (1.0f / 65536.0f) + e * (1.0f / 4096.0f));
+    b.ux = ux;
+    b.uy = uy;
+    b.uz = uz;
+    b.g = g == g ? g : -INFINITY;
+    return b;
+}
+
 __global__ void k_fill_u32g(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -416,7 +491,8 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
                                                     const SeedBox* __restrict__ boxes,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint64_t* __restrict__ total_ptr, uint64_t* keys,
-                                                    DevFlags* flags, unsigned long long* evals) {
+                                                    uint32_t* __restrict__ dirs, DevFlags* flags,
+                                                    unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
     __shared__ uint64_t s_off[SEED_CHUNK];
     __shared__ uint64_t s_t0, s_t1;
@@ -462,13 +538,14 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
                 uint32_t z = sb.mn[2] + (uint32_t)(r % sb.ext[2]);
                 // the 48-byte copy k_seed_boxes wrote: three loads instead of six, the same values
                 const TriRec tr = expand_leaf(R.ltris[t - R.tri_off]);
-                float d = cell_dist<SIGN>(R.g, x, y, z, tr);
+                float dx, dy, dz;
+                float d = cell_dist_dir<SIGN>(R.g, x, y, z, tr, dx, dy, dz);
                 n_evals++;
                 if (d != d) atomicOr(&flags[R.flag].nan_distance, 1u);
                 if (key_valid(d)) {
                     uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
-                    uint64_t* slot = &keys[R.cell_off + cell_index(R.g, x, y, z)];
-                    key_lower<SIGN>(slot, key, *slot);
+                    const uint64_t ci = R.cell_off + cell_index(R.g, x, y, z);
+                    if (key_lower<SIGN>(&keys[ci], key, keys[ci])) dirs[ci] = dir_pack(dx, dy, dz);
                 }
             });
         }
@@ -723,7 +800,8 @@ struct PropAcc {
 // request q to its mesh-local triangle tri, kept if its key is below the
 // neighbour's current key.
 template <int MODE>
-__device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, uint8_t* __restrict__ changed,
+__device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, uint32_t* __restrict__ dirs,
+                                              uint8_t* __restrict__ changed,
                                               uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
                                               DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
                                               uint32_t q, uint64_t cur, PropAcc& acc) {
@@ -759,6 +837,7 @@ __device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, 
     const uint64_t key = make_key<SIGN>(d, tri);
     const uint64_t ci = R.cell_off + cell_index(R.g, x, y, z);
     if (key_lower<SIGN>(&keys[ci], key, cur)) {
+        dirs[ci] = dir_pack(dx, dy, dz);
         changed[ci] = 1;
         tile_changed[ci / COMPACT_TILE] = 1;
         if (mode_batch<MODE> && q != acc.req && acc.imin != UINT32_MAX) {  // the lane moved on to another request
@@ -775,7 +854,7 @@ __device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, 
 // once through the scalar cache.
 template <int MODE>
 __device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
-                                          uint64_t* keys, uint8_t* __restrict__ changed,
+                                          uint64_t* keys, uint32_t* __restrict__ dirs, uint8_t* __restrict__ changed,
                                           uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
                                           DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
                                           uint32_t q, uint64_t cur, PropAcc& acc) {
@@ -783,45 +862,27 @@ __device__ __forceinline__ void prop_eval(const ReqView<mode_batch<MODE>>& reqs,
         const uint32_t q0 = __builtin_amdgcn_readfirstlane(q);
         if (__ballot(q != q0) == 0) {
             const GridReq R = wave_req(reqs.table, q0);
-            prop_eval_req<MODE>(R, keys, changed, tile_changed, min_next, flags, x, y, z, tri, q0, cur, acc);
+            prop_eval_req<MODE>(R, keys, dirs, changed, tile_changed, min_next, flags, x, y, z, tri, q0, cur, acc);
         } else {
-            prop_eval_req<MODE>(reqs.table[q], keys, changed, tile_changed, min_next, flags, x, y, z, tri, q, cur,
-                                acc);
+            prop_eval_req<MODE>(reqs.table[q], keys, dirs, changed, tile_changed, min_next, flags, x, y, z, tri, q,
+                                cur, acc);
         }
     } else {
-        prop_eval_req<MODE>(reqs.one, keys, changed, tile_changed, min_next, flags, x, y, z, tri, 0u, cur, acc);
+        prop_eval_req<MODE>(reqs.one, keys, dirs, changed, tile_changed, min_next, flags, x, y, z, tri, 0u, cur, acc);
     }
 }
 
-// Tries each item's triangle on the item's 26 neighbours: one lane per pair
-// tests bounds and the neighbour's current key, the pairs left go through the
-// wave's queue to prop_eval. A strictly smaller key is stored with atomicMin
-// and flags the cell for a later round. The reads before the atomic are plain
-// (cacheable) loads: keys only decrease, so a stale value can only cost an
-// unneeded evaluation or atomic. Improved distances are folded into each
-// request's min_next words; min_cur and taken_next (read by this round's
-// compaction, written by the next one) are reset. The frontier size goes to
-// *total_out and, when host_probe is not null, to host_probe[0] in pinned host
-// memory, followed (after a system-scope fence) by host_seq in host_probe[1]:
-// the host reads the size once it sees the sequence number.
+// The start of a propagation launch: prefix[c] = items in the classes before
+// c; min_cur and taken_next (read by this round's compaction, written by the
+// next one) are reset; block 0 writes the frontier size to *total_out and,
+// when host_probe is not null, to host_probe[0] in pinned host memory,
+// followed (after a system-scope fence) by host_seq in host_probe[1]: the host
+// reads the size once it sees the sequence number. Ends with a block barrier.
 template <int MODE>
-__global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
-                                                            const uint32_t* __restrict__ ctr_cur,
-                                                            ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
-                                                            uint64_t* keys, uint8_t* __restrict__ changed,
-                                                            uint8_t* __restrict__ tile_changed,
-                                                            uint32_t* __restrict__ min_next,
-                                                            uint32_t* __restrict__ min_cur,
-                                                            uint32_t* __restrict__ taken_next,
-                                                            uint32_t* __restrict__ total_out,
-                                                            uint32_t* __restrict__ host_probe, uint32_t host_seq,
-                                                            uint32_t item_base,
-                                                            DevFlags* flags,
-                                                            unsigned long long* evals) {
-    constexpr int SIGN = mode_sign<MODE>;
-    // prefix[c]: items in the classes before c
-    __shared__ uint32_t prefix[NCLASS + 1];
-    __shared__ PropQueue<mode_batch<MODE>> Q;
+__device__ __forceinline__ void prop_prologue(const uint32_t* __restrict__ ctr_cur, uint32_t* prefix, uint32_t nreq,
+                                              uint32_t* __restrict__ min_cur, uint32_t* __restrict__ taken_next,
+                                              uint32_t* __restrict__ total_out, uint32_t* __restrict__ host_probe,
+                                              uint32_t host_seq) {
     if (threadIdx.x < 64u) {
         const uint32_t c = ctr_cur[threadIdx.x * WORD_STRIDE];
         uint32_t x = c;
@@ -847,6 +908,39 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
         taken_next[w * WORD_STRIDE] = 0u;
     }
     __syncthreads();
+}
+
+// Tries each item's triangle on the item's 26 neighbours: one lane per pair
+// tests bounds and the neighbour's current key, the pairs left go through the
+// wave's queue to prop_eval. A strictly smaller key is stored with atomicMin
+// and flags the cell for a later round. The reads before the atomic are plain
+// (cacheable) loads: keys only decrease, so a stale value can only cost an
+// unneeded evaluation or atomic. Improved distances are folded into each
+// request's min_next words; min_cur and taken_next (read by this round's
+// compaction, written by the next one) are reset. The frontier size goes to
+// *total_out and, when host_probe is not null, to host_probe[0] in pinned host
+// memory, followed (after a system-scope fence) by host_seq in host_probe[1]:
+// the host reads the size once it sees the sequence number.
+template <int MODE>
+__global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierItem* __restrict__ items, uint32_t region,
+                                                            const uint32_t* __restrict__ ctr_cur,
+                                                            ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
+                                                            uint64_t* keys, uint32_t* __restrict__ dirs,
+                                                            uint8_t* __restrict__ changed,
+                                                            uint8_t* __restrict__ tile_changed,
+                                                            uint32_t* __restrict__ min_next,
+                                                            uint32_t* __restrict__ min_cur,
+                                                            uint32_t* __restrict__ taken_next,
+                                                            uint32_t* __restrict__ total_out,
+                                                            uint32_t* __restrict__ host_probe, uint32_t host_seq,
+                                                            uint32_t item_base,
+                                                            DevFlags* flags,
+                                                            unsigned long long* evals) {
+    constexpr int SIGN = mode_sign<MODE>;
+    // prefix[c]: items in the classes before c
+    __shared__ uint32_t prefix[NCLASS + 1];
+    __shared__ PropQueue<mode_batch<MODE>> Q;
+    prop_prologue<MODE>(ctr_cur, prefix, nreq, min_cur, taken_next, total_out, host_probe, host_seq);
     // this launch's items: [item_base, item_base + PROP_CHUNK_ITEMS) of the
     // frontier (one launch covers the frontier of grids below 165M cells;
     // larger ones run several launches per round), so the 32-bit work index
@@ -918,16 +1012,156 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
             qn -= 64u;
             const uint32_t e = qn + lane;
             const uint64_t c = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
-            prop_eval<MODE>(reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][e], Q.y[wave][e],
+            prop_eval<MODE>(reqs, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][e], Q.y[wave][e],
                             Q.z[wave][e], Q.tri[wave][e], mode_batch<MODE> ? Q.req[wave][e] : 0u, c, acc);
             wave_lds_sync();
         }
     }
     if (lane < qn) {
         const uint64_t c = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
-        prop_eval<MODE>(reqs, keys, changed, tile_changed, min_next, flags, Q.x[wave][lane],
+        prop_eval<MODE>(reqs, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][lane],
                         Q.y[wave][lane], Q.z[wave][lane], Q.tri[wave][lane],
                         mode_batch<MODE> ? Q.req[wave][lane] : 0u, c, acc);
+    }
+    if (mode_batch<MODE>) fold_req_min<MODE>(min_next, acc.req, acc.imin, blockIdx.x);
+    else block_min_u32(req_word<MODE>(min_next, 0, blockIdx.x), acc.imin);
+    if (evals) block_count(evals, acc.n_evals);
+}
+
+// k_frontier_propagate for large rounds, item-major: one lane per frontier
+// item walks the item's 26 neighbours, so the supporting-plane bound
+// (plane_bound) is computed once per item and skips most of the neighbours
+// that cannot improve before their distance is evaluated; the item is read
+// once instead of once per neighbour. A skipped pair cannot improve its
+// neighbour, so the keys end exactly as k_frontier_propagate leaves them; the
+// host picks this kernel for rounds of at least grid_item_major_min items,
+// where 26 dependent steps per lane are hidden by many waves. Same arguments
+// and round bookkeeping as k_frontier_propagate.
+template <int MODE>
+__global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const FrontierItem* __restrict__ items,
+                                                                  uint32_t region, const uint32_t* __restrict__ ctr_cur,
+                                                                  ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
+                                                                  uint64_t* keys, uint32_t* __restrict__ dirs,
+                                                                  uint8_t* __restrict__ changed,
+                                                                  uint8_t* __restrict__ tile_changed,
+                                                                  uint32_t* __restrict__ min_next,
+                                                                  uint32_t* __restrict__ min_cur,
+                                                                  uint32_t* __restrict__ taken_next,
+                                                                  uint32_t* __restrict__ total_out,
+                                                                  uint32_t* __restrict__ host_probe, uint32_t host_seq,
+                                                                  uint32_t item_base, DevFlags* flags,
+                                                                  unsigned long long* evals) {
+    constexpr int SIGN = mode_sign<MODE>;
+    __shared__ uint32_t prefix[NCLASS + 1];
+    __shared__ PropQueue<mode_batch<MODE>> Q;
+    // each lane's item and bound, read back at every neighbour so that they
+    // need not stay in registers across the queue's evaluations
+    __shared__ uint4 s_item[256];
+    __shared__ float4 s_bound[256];
+    prop_prologue<MODE>(ctr_cur, prefix, nreq, min_cur, taken_next, total_out, host_probe, host_seq);
+    const uint32_t nfront = prefix[NCLASS];
+    const uint32_t work = nfront > item_base ? min(nfront - item_base, PROP_CHUNK_ITEMS) : 0u;
+    // each XCD takes one contiguous eighth of the items (in spatial order)
+    const uint32_t chunk = (work + XCDS - 1) / XCDS;
+    const uint32_t j0 = (blockIdx.x % XCDS) * chunk, j1 = min(work, j0 + chunk);
+    const uint32_t stride = gridDim.x / XCDS * blockDim.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    PropAcc acc;
+    uint32_t qn = 0;  // queued pairs (the same value in every lane of the wave)
+    for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
+        const uint32_t j = base + lane;
+        const bool live = j < j1;
+        uint32_t tri = 0, q = 0, ix = 0, iy = 0, iz = 0;
+        if (live) {
+            const uint32_t f = item_base + j;
+            uint32_t cls = 0;  // the last class whose prefix is <= f (a non-empty one)
+#pragma unroll
+            for (uint32_t step = NCLASS / 2; step; step >>= 1)
+                if (prefix[cls + step] <= f) cls += step;
+            read_item<MODE>(items[(uint64_t)cls * region + (f - prefix[cls])], tri, q, ix, iy, iz);
+        }
+        auto walk = [&](const GridReq& R, bool mine) {
+            if (mine) {
+                const float max_size = fmaxf(fmaxf(fabsf(R.g.size[0]), fabsf(R.g.size[1])), fabsf(R.g.size[2]));
+                const PlaneBound pb = plane_bound(dirs[R.cell_off + cell_index(R.g, ix, iy, iz)],
+                                                  cell_coord(R.g, 0, ix), cell_coord(R.g, 1, iy),
+                                                  cell_coord(R.g, 2, iz), expand_leaf(R.ltris[tri]), max_size);
+                s_bound[threadIdx.x] = make_float4(pb.ux, pb.uy, pb.uz, pb.g);  // only this lane reads it
+                s_item[threadIdx.x] = make_uint4(ix, iy, iz, tri);
+            }
+            for (uint32_t k = 0; k < 27u; ++k) {
+                if (k == 13u) continue;  // the cell itself never improves
+                bool keep = false;
+                uint32_t x = 0, y = 0, z = 0, t = 0;
+                uint64_t cur = 0;
+                const int ox = (int)(k / 9u) - 1, oy = (int)((k / 3u) % 3u) - 1, oz = (int)(k % 3u) - 1;
+                if (mine) {
+                    const uint4 it = s_item[threadIdx.x];
+                    t = it.w;
+                    x = it.x + (uint32_t)ox;  // -1 wraps past the axis length
+                    y = it.y + (uint32_t)oy;
+                    z = it.z + (uint32_t)oz;
+                    if (x < R.g.n[0] && y < R.g.n[1] && z < R.g.n[2]) {
+                        cur = keys[R.cell_off + cell_index(R.g, x, y, z)];
+                        // a neighbour already holding this triangle holds exactly the key it would give
+                        keep = cur == KEY_EMPTY || key_tri(cur, SIGN) != t;
+                        if (keep && cur != KEY_EMPTY) {
+                            // |d| must exceed the key's distance to come first (in Normal
+                            // mode, past a negative value it approximately equals, as in
+                            // prop_eval_req)
+                            uint32_t cb = (uint32_t)(cur >> 32);
+                            if (SIGN == SDFX_SIGN_NORMAL && ((cur >> 31) & 1ull))
+                                cb = max(fbits(bitsf(cb) + 2e-6f), cb + 2u);
+                            const float4 pb = s_bound[threadIdx.x];
+                            const float lb = pb.w + (pb.x * ((float)ox * R.g.size[0]) + pb.y * ((float)oy * R.g.size[1]) +
+                                                     pb.z * ((float)oz * R.g.size[2]));
+                            if (lb > bitsf(cb)) keep = false;
+                        }
+                    }
+                }
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t pos = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    Q.x[wave][pos] = x;
+                    Q.y[wave][pos] = y;
+                    Q.z[wave][pos] = z;
+                    Q.tri[wave][pos] = t;
+                    if constexpr (mode_batch<MODE>) Q.req[wave][pos] = q;
+                    Q.cur_lo[wave][pos] = (uint32_t)cur;
+                    Q.cur_hi[wave][pos] = (uint32_t)(cur >> 32);
+                }
+                qn += (uint32_t)__popcll(m);
+                wave_lds_sync();
+                if (qn >= 64u) {  // evaluate the newest 64 queued pairs
+                    qn -= 64u;
+                    const uint32_t e = qn + lane;
+                    const uint64_t cv = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
+                    prop_eval<MODE>(reqs, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][e],
+                                    Q.y[wave][e], Q.z[wave][e], Q.tri[wave][e],
+                                    mode_batch<MODE> ? Q.req[wave][e] : 0u, cv, acc);
+                    wave_lds_sync();
+                }
+            }
+        };
+        if constexpr (mode_batch<MODE>) {
+            // one walk per request among the wave's items (nearly always one:
+            // items lie request by request), each with every lane taking part
+            // and the request read once through the scalar cache
+            uint64_t pending = __ballot(live);
+            while (pending) {
+                const uint32_t q0 = __builtin_amdgcn_readlane(q, __ffsll((unsigned long long)pending) - 1);
+                const bool mine = live && q == q0;
+                walk(wave_req(reqs.table, q0), mine);
+                pending &= ~__ballot(mine);
+            }
+        } else {
+            walk(reqs.one, live);
+        }
+    }
+    if (lane < qn) {
+        const uint64_t cv = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
+        prop_eval<MODE>(reqs, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][lane], Q.y[wave][lane],
+                        Q.z[wave][lane], Q.tri[wave][lane], mode_batch<MODE> ? Q.req[wave][lane] : 0u, cv, acc);
     }
     if (mode_batch<MODE>) fold_req_min<MODE>(min_next, acc.req, acc.imin, blockIdx.x);
     else block_min_u32(req_word<MODE>(min_next, 0, blockIdx.x), acc.imin);
@@ -1356,6 +1590,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // then per request its band minima and its taken-cell counts; the two
     // frontier totals and the per-request far-field flags; the ray-parity bits
     uint64_t* keys = A.alloc<uint64_t>(ncells);
+    uint32_t* dirs = A.alloc<uint32_t>(ncells);  // direction words: any value is valid, left as they are
     const uint64_t cls_words = (uint64_t)NCLASS * WORD_STRIDE;
     const uint64_t req_words = (uint64_t)nreq * req_spread<MODE> * WORD_STRIDE;
     const uint64_t bit_words = SIGN == SDFX_SIGN_RAYCAST ? words_total[0] + words_total[1] + words_total[2] : 0;
@@ -1403,7 +1638,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
                     (uint64_t)reqs[0].ncells, keys, d_flags + reqs[0].flag);
     } else {
     exclusive_scan_u64(counts, counts, nt, total, A);
-    SDFX_LAUNCH(k_seed_pairs<MODE>, dim3(8192), dim3(256), 0, s, nt, rv, tri_req, boxes, counts, total, keys,
+    SDFX_LAUNCH(k_seed_pairs<MODE>, dim3(8192), dim3(256), 0, s, nt, rv, tri_req, boxes, counts, total, keys, dirs,
                 d_flags, evals);
     SDFX_HIP(hipGetLastError());
 
@@ -1446,15 +1681,29 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     auto enqueue_batch = [&](int slot) {
         const uint32_t seq = probe.arm(slot);
         const unsigned blocks = last_taken < 2048u ? std::min(prop_blocks, 256u) : prop_blocks;
+        // large rounds propagate item-major (the same keys, fewer evaluations);
+        // before the first frontier size is known, a sixteenth of the cells
+        // stands for it (grid_item_major_min = 0 forces it everywhere)
+        const uint64_t round_cells = last_taken == UINT32_MAX ? ncells / 16 : last_taken;
+        const int64_t im_min = tuning().grid_item_major_min;
+        const bool item_major = im_min == 0 || round_cells >= (uint64_t)im_min;
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
                         taken[cur ^ 1], taken[cur], released, (int)SIGN);
-            for (uint32_t k = 0; k < prop_chunks; ++k)
-                SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(blocks), dim3(256), 0, s, items, region, ctr[cur],
-                            rv, nreq, keys, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
-                            totals + cur, b == batch - 1 && k + 1 == prop_chunks ? probe.probe(slot) : nullptr, seq,
-                            k * PROP_CHUNK_ITEMS, d_flags, evals ? evals + seed_slots : nullptr);
+            for (uint32_t k = 0; k < prop_chunks; ++k) {
+                uint32_t* probe_w = b == batch - 1 && k + 1 == prop_chunks ? probe.probe(slot) : nullptr;
+                if (item_major)
+                    SDFX_LAUNCH(k_frontier_propagate_items<MODE>, dim3(blocks), dim3(256), 0, s, items, region,
+                                ctr[cur], rv, nreq, keys, dirs, changed, tile_changed, minw[cur ^ 1], minw[cur],
+                                taken[cur ^ 1], totals + cur, probe_w, seq, k * PROP_CHUNK_ITEMS, d_flags,
+                                evals ? evals + seed_slots : nullptr);
+                else
+                    SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(blocks), dim3(256), 0, s, items, region, ctr[cur],
+                                rv, nreq, keys, dirs, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
+                                totals + cur, probe_w, seq, k * PROP_CHUNK_ITEMS, d_flags,
+                                evals ? evals + seed_slots : nullptr);
+            }
             cur ^= 1;
         }
         SDFX_HIP(hipGetLastError());

@@ -150,6 +150,10 @@ struct Tuning {
     int64_t grid_prop_blocks;
     int64_t lbvh_visit_budget;  // node visits after which a nearest search moves to the per-wave pass (0: by mesh size)
     int64_t grid_batch_prop_blocks;  // grid_prop_blocks for launches that serve several requests
+    // rounds after one that took at least this many cells propagate item-major
+    // (k_frontier_propagate_items, measured faster on batches whose rounds
+    // reach millions of items, slower on a 128^3 grid); 0: every round
+    int64_t grid_item_major_min;
 };
 Tuning& tuning();
 

@@ -207,7 +207,7 @@ namespace sdfx {
 Tuning& tuning() {
     // 2^26 pairs: about 10k queries against 6k triangles, where the exhaustive
     // kernel finishes well before an LBVH build and traversal would.
-    static Tuning t{int64_t(1) << 26, 4096, 0, 2048};
+    static Tuning t{int64_t(1) << 26, 4096, 0, 2048, 1 << 20};
     return t;
 }
 
@@ -326,6 +326,7 @@ int64_t* tuning_slot(const char* key) {
     if (k == "grid_prop_blocks") return &sdfx::tuning().grid_prop_blocks;
     if (k == "lbvh_visit_budget") return &sdfx::tuning().lbvh_visit_budget;
     if (k == "grid_batch_prop_blocks") return &sdfx::tuning().grid_batch_prop_blocks;
+    if (k == "grid_item_major_min") return &sdfx::tuning().grid_item_major_min;
     return nullptr;
 }
 }  // namespace
@@ -337,6 +338,7 @@ int sdfx_set_tuning(const char* key, int64_t value) {
         (value < 8 || value > 65536))
         return -1;
     if (std::string(key) == "lbvh_visit_budget" && value != 0 && (value < 16 || value > INT32_MAX)) return -1;
+    if (std::string(key) == "grid_item_major_min" && value < 0) return -1;
     *slot = value;
     return 0;
 }

@@ -67,7 +67,8 @@ def test_tuning_knobs_and_ranges():
     from mesh_to_sdf_amd import _lib
     L = _lib.lib()
     for key, good, bad in (("exhaustive_max_pairs", 12345, None), ("grid_prop_blocks", 512, 4),
-                           ("lbvh_visit_budget", 64, 15), ("grid_batch_prop_blocks", 1024, 65537)):
+                           ("lbvh_visit_budget", 64, 15), ("grid_batch_prop_blocks", 1024, 65537),
+                           ("grid_item_major_min", 4096, -1)):
         old = L.sdfx_get_tuning(key.encode())
         assert old >= 0 or key == "exhaustive_max_pairs"
         assert L.sdfx_set_tuning(key.encode(), good) == 0 and L.sdfx_get_tuning(key.encode()) == good
