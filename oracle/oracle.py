@@ -30,7 +30,8 @@ TOPO_LIST, TOPO_STRIP = 0, 1
 SIGN_RAYCAST, SIGN_NORMAL = 0, 1
 ACCEL_NONE, ACCEL_BVH, ACCEL_RTREE, ACCEL_RTREE_BVH = 0, 1, 2, 3
 GRID_SEQUENTIAL, GRID_THREADED, GRID_FRONTIER, GRID_EXACT = 0, 1, 2, 3
-FAULT_NONE, FAULT_SKIP_DIRECTION, FAULT_DROP_DEFERRED, FAULT_PACKED_ORDER = 0, 1, 2, 3
+FAULT_NONE, FAULT_SKIP_DIRECTION, FAULT_DROP_DEFERRED, FAULT_PACKED_ORDER, FAULT_BOUND_MARGIN_SIGN = 0, 1, 2, 3, 4
+BOUND_OFF, BOUND_EXACT_DIRECTION, BOUND_RANDOM_DIRECTION = 0, 1, 2
 NO_TRIANGLE = 0xFFFFFFFF
 
 _f = ctypes.POINTER(ctypes.c_float)
@@ -95,6 +96,8 @@ def lib() -> ctypes.CDLL:
             "orc_grid_fixpoint_check": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64, ctypes.c_int, _f,
                                                        _f, _u64, ctypes.c_int, _f, _u32, ctypes.c_int, _u64]),
             "orc_set_fault": (None, [ctypes.c_int]),
+            "orc_set_bound_check": (None, [ctypes.c_int]),
+            "orc_bound_check_counts": (None, [_u64]),
             "orc_nearest_bvh": (None, [_f, _u32, ctypes.c_uint64, _f, ctypes.c_uint64, _f, _u32, ctypes.c_int]),
             "orc_query_vector_to_closest_point": (ctypes.c_int, [_f, ctypes.c_uint64, _u32, ctypes.c_uint64,
                                                                  ctypes.c_int, _f, ctypes.c_uint64, _f, _u8,
@@ -310,6 +313,19 @@ def grid_fixpoint_check(vertices, indices, topology, first_cell, cell_size, cell
 def set_fault(kind):
     """Test-only: inject a fault into the GRID_FRONTIER schedule (FAULT_*)."""
     lib().orc_set_fault(kind)
+
+
+def set_bound_check(mode):
+    """Test-only: check the device's supporting-plane skip on every
+    GRID_FRONTIER pair (orc_set_bound_check; BOUND_*); resets the counts."""
+    lib().orc_set_bound_check(mode)
+
+
+def bound_check_counts():
+    """Pairs checked, pairs the bound skips, skipped pairs that would improve."""
+    c = np.zeros(3, np.uint64)
+    lib().orc_bound_check_counts(c.ctypes.data_as(_u64))
+    return dict(zip(("checked", "skipped", "wrong"), map(int, c)))
 
 
 def query_vector_to_closest_point(vertices, indices, topology, queries, n_threads=None):

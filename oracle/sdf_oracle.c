@@ -1053,6 +1053,57 @@ static inline int key_lower(uint64_t* p, uint64_t v, int sign) {
  * tests can show that the fixpoint check catches a schedule that is wrong. */
 void orc_set_fault(int kind) { g_fault = kind; }
 
+/* ---------------------------------------------- supporting-plane skip check */
+static int g_bound_mode;
+static uint64_t g_bound_counts[3];
+void orc_set_bound_check(int mode) {
+    g_bound_mode = mode;
+    g_bound_counts[0] = g_bound_counts[1] = g_bound_counts[2] = 0;
+}
+void orc_bound_check_counts(uint64_t* counts) {
+    for (int i = 0; i < 3; ++i) counts[i] = g_bound_counts[i];
+}
+/* dir_pack: the vector over its largest component, three signed 10-bit fields */
+static uint32_t bound_dir_pack(float dx, float dy, float dz) {
+    float m = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    if (!(m > 0.0f) || !(m <= 3.4e38f)) return 0u;
+    float sc = 511.0f / m;
+    int qx = (int)rintf(dx * sc), qy = (int)rintf(dy * sc), qz = (int)rintf(dz * sc);
+    return ((uint32_t)qx & 1023u) | (((uint32_t)qy & 1023u) << 10) | (((uint32_t)qz & 1023u) << 20);
+}
+/* plane_bound: u from the word, normalised; no bound for a zero word or a
+ * sliver; g = u.c - max_i u.v_i - (2^-16 largest coordinate + 2^-12 longest edge) */
+static void plane_bound_c(uint32_t w, v3 c, v3 A, v3 B, v3 C, float max_size, float* u4) {
+    u4[0] = u4[1] = u4[2] = 0.0f;
+    u4[3] = -INFINITY;
+    float ux = (float)(((int32_t)(w << 22)) >> 22), uy = (float)(((int32_t)(w << 12)) >> 22),
+          uz = (float)(((int32_t)(w << 2)) >> 22);
+    float l2 = ux * ux + uy * uy + uz * uz;
+    v3 ab = vsub(B, A), ac = vsub(C, A), bc = vsub(C, B), n = vcross(ab, ac);
+    float ab2 = vdot(ab, ab), ac2 = vdot(ac, ac), bc2 = vdot(bc, bc), n2 = vdot(n, n);
+    int degen = veq(A, B) || veq(B, C) || veq(A, C);
+    if (!(l2 > 0.0f) || (!degen && !(n2 >= (ab2 * ac2) * (1.0f / 256.0f)))) return;
+    float inv = 1.0f / sqrtf(l2);
+    ux *= inv;
+    uy *= inv;
+    uz *= inv;
+    v3 u = mk(ux, uy, uz);
+    float h = fmaxf(vdot(u, A), fmaxf(vdot(u, B), vdot(u, C)));
+    float sc = fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fabsf(c.z));
+    sc = fmaxf(sc, fmaxf(fmaxf(fabsf(A.x), fabsf(A.y)), fabsf(A.z)));
+    sc = fmaxf(sc, fmaxf(fmaxf(fabsf(B.x), fabsf(B.y)), fabsf(B.z)));
+    sc = fmaxf(sc, fmaxf(fmaxf(fabsf(C.x), fabsf(C.y)), fabsf(C.z)));
+    sc = sc + max_size;
+    float e = sqrtf(fmaxf(ab2, fmaxf(ac2, bc2)));
+    float margin = sc * (1.0f / 65536.0f) + e * (1.0f / 4096.0f);
+    if (g_fault == ORC_FAULT_BOUND_MARGIN_SIGN) margin = -margin;
+    float g = (vdot(u, c) - h) - margin;
+    u4[0] = ux;
+    u4[1] = uy;
+    u4[2] = uz;
+    u4[3] = g == g ? g : -INFINITY;
+}
+
 static void frontier_seed_triangle(const ogrid* g, const float* verts, const uint32_t* tris, uint32_t t, int sign,
                                    uint64_t* keys, int* nan_seen) {
     v3 lo, hi;
@@ -1130,12 +1181,28 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
         }
         rounds++;
         prev_taken = nf;
-#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 256) reduction(| : nan_seen) reduction(+ : n_evals)
+        uint64_t bc_checked = 0, bc_skipped = 0, bc_wrong = 0;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 256) reduction(| : nan_seen) \
+    reduction(+ : n_evals, bc_checked, bc_skipped, bc_wrong)
         for (int64_t f = 0; f < (int64_t)nf; ++f) {
             uint64_t ci = fr_cell[f];
             uint32_t t = fr_tri[f];
             int64_t cz = (int64_t)(ci % g->n[2]), cy = (int64_t)((ci / g->n[2]) % g->n[1]),
                     cx = (int64_t)(ci / (g->n[1] * g->n[2]));
+            float bound[4] = {0.0f, 0.0f, 0.0f, -INFINITY};
+            if (g_bound_mode) {
+                v3 c = cell_center(g, (uint64_t)cx, (uint64_t)cy, (uint64_t)cz);
+                v3 A = TRI_A(t), B = TRI_B(t), C = TRI_C(t);
+                uint32_t w;
+                if (g_bound_mode == 1) {
+                    v3 q = closest_triangle(c, A, B, C);
+                    w = bound_dir_pack(c.x - q.x, c.y - q.y, c.z - q.z);
+                } else {
+                    uint64_t hsh = (ci + 0x9E3779B97F4A7C15ull * (uint64_t)(f + 1)) * 0xBF58476D1CE4E5B9ull;
+                    w = (uint32_t)(hsh >> 32);
+                }
+                plane_bound_c(w, c, A, B, C, max_size, bound);
+            }
             for (int dx = -1; dx <= 1; ++dx)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dz = -1; dz <= 1; ++dz) {
@@ -1146,14 +1213,43 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
                             z >= (int64_t)g->n[2])
                             continue;
                         uint64_t cell[3] = {(uint64_t)x, (uint64_t)y, (uint64_t)z};
+                        uint64_t ni = cell_idx(g, cell[0], cell[1], cell[2]);
+                        int skip = 0;
+                        uint64_t cur = 0;
+                        if (g_bound_mode) {
+                            cur = __atomic_load_n(&keys[ni], __ATOMIC_RELAXED);
+                            if (cur != KEY_EMPTY && key_tri(cur, sign) != t) {
+                                bc_checked++;
+                                uint32_t cb = (uint32_t)(cur >> 32);
+                                if (sign == ORC_SIGN_NORMAL && ((cur >> 31) & 1ull)) {
+                                    float cf, r;
+                                    uint32_t rb;
+                                    memcpy(&cf, &cb, 4);
+                                    r = cf + 2e-6f;
+                                    memcpy(&rb, &r, 4);
+                                    cb = rb > cb + 2u ? rb : cb + 2u;
+                                }
+                                float thr;
+                                memcpy(&thr, &cb, 4);
+                                float lb = bound[3] + (bound[0] * ((float)dx * g->size.x) + bound[1] * ((float)dy * g->size.y) +
+                                                       bound[2] * ((float)dz * g->size.z));
+                                if (lb > thr) {
+                                    skip = 1;
+                                    bc_skipped++;
+                                }
+                            }
+                        }
                         float d = cell_distance(g, verts, tris, t, cell, sign);
                         n_evals++;
                         if (d != d) nan_seen = 1;
-                        uint64_t ni = cell_idx(g, cell[0], cell[1], cell[2]);
+                        if (skip && key_valid(d) && key_before(make_key(d, t, sign), cur, sign)) bc_wrong++;
                         if (key_valid(d) && key_lower(&keys[ni], make_key(d, t, sign), sign))
                             __atomic_store_n(&changed[ni], 1, __ATOMIC_RELAXED);
                     }
         }
+        g_bound_counts[0] += bc_checked;
+        g_bound_counts[1] += bc_skipped;
+        g_bound_counts[2] += bc_wrong;
     }
     for (uint64_t i = 0; i < ncells; ++i) dist_arr[i] = key_value(keys[i], sign);
     if (tri_out)

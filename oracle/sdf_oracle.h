@@ -157,8 +157,23 @@ int orc_grid_fixpoint_check(const float* vertices, uint64_t n_vertices, const ui
  * ORC_FAULT_PACKED_ORDER orders Normal-mode values by the packed key alone
  * (|d|, then negative after positive) instead of compare_distances, which
  * prefers a positive value over a negative one it approximately equals. */
-enum { ORC_FAULT_NONE = 0, ORC_FAULT_SKIP_DIRECTION = 1, ORC_FAULT_DROP_DEFERRED = 2, ORC_FAULT_PACKED_ORDER = 3 };
+enum { ORC_FAULT_NONE = 0, ORC_FAULT_SKIP_DIRECTION = 1, ORC_FAULT_DROP_DEFERRED = 2, ORC_FAULT_PACKED_ORDER = 3,
+       ORC_FAULT_BOUND_MARGIN_SIGN = 4 };
 void orc_set_fault(int kind);
+
+/* Test-only check of the device propagation's supporting-plane skip
+ * (plane_bound in sdfx_grid.hip), restated in FP32 with the same operations.
+ * With mode 1 (direction words as improvements write them: the quantised
+ * vector from the item cell's closest point on its triangle to its centre) or
+ * mode 2 (a pseudo-random word per item), every ORC_GRID_FRONTIER propagation
+ * pair whose neighbour holds a key from another triangle is tested: counts[0]
+ * such pairs, counts[1] those the bound skips, counts[2] skipped pairs whose
+ * distance would still have lowered the neighbour's key (must be 0; the
+ * schedule itself evaluates every pair, so results are unchanged). Mode 0:
+ * off. ORC_FAULT_BOUND_MARGIN_SIGN adds the bound's rounding margin instead of
+ * subtracting it, which the check must catch. */
+void orc_set_bound_check(int mode);
+void orc_bound_check_counts(uint64_t* counts);
 
 /* Vectors from each query to its closest point on the nearest triangle, plus the
  * 3-ray inside vote (generate/generic/rtree_bvh.rs:217-276). */

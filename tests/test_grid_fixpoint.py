@@ -174,3 +174,93 @@ def test_normal_mode_ties_follow_compare_distances(oracle, suzanne, mesh, n):
         tie = afr == ah1
         assert np.all(fr[bad][tie] > 0)
         assert np.all(afr[~tie] < ah1[~tie]) and np.all(afr >= exact)
+
+
+def _slivers(seed, n=200):
+    """Nearly collinear triangles (third vertex within 1e-7..1e-2 of the line
+    through the first two, inside or past the segment) offset up to 10^4 from
+    the origin, with a random grid around them."""
+    g = np.random.default_rng(seed)
+    off = np.float32(10.0 ** g.uniform(0, 4))
+    a = g.uniform(-1, 1, (n, 3))
+    d = g.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    length = g.uniform(0.01, 0.5, (n, 1))
+    b = a + d * length
+    eps = 10.0 ** g.uniform(-7, -2, (n, 1))
+    c = a + d * length * g.uniform(-0.5, 1.5, (n, 1)) + g.normal(size=(n, 3)) * eps * length
+    v = (np.concatenate([a, b, c]) + off).astype(np.float32)
+    f = np.stack([np.arange(n), np.arange(n) + n, np.arange(n) + 2 * n], 1).astype(np.uint32)
+    lo, hi = v.min(0), v.max(0)
+    cnt = [int(x) for x in g.integers(16, 40, 3)]
+    size = ((hi - lo) * 1.2 / np.array(cnt)).astype(np.float32)
+    first = (lo - 0.1 * (hi - lo) + size / 2).astype(np.float32)
+    return v, f.reshape(-1), first, size, cnt
+
+
+def _bound_cases(suzanne):
+    from test_gpu_random_meshes import random_grid, soup
+    sv, sf = suzanne
+    yield "torus", *S.torus(1.0, 0.35, 48, 24, 0.002), (24, 20, 28), 0
+    yield "noise sphere", *S.noise_sphere(3, 0.05), (30, 30, 30), 1
+    yield "suzanne", np.asarray(sv, np.float32), np.asarray(sf, np.uint32), (32, 32, 32), 1
+    for seed in range(12):
+        v, f, _ = soup(seed)
+        first, size, count = random_grid(np.random.default_rng(seed + 500), v)
+        yield f"soup {seed}", v, f.reshape(-1), (first, size, [min(48, 2 * c) for c in count]), seed & 1
+    for seed in range(12):
+        v, f, first, size, count = _slivers(seed)
+        yield f"slivers {seed}", v, f, (first, size, count), seed & 1
+
+
+def _run_bound(O, v, f, grid, sign, mode):
+    if len(grid) == 3 and np.ndim(grid[0]) == 0:
+        first, size, count = S.grid_for(v, grid)
+    else:
+        first, size, count = grid
+    O.set_bound_check(mode)
+    try:
+        s, _, _ = O.generate_grid_sdf(v, f, O.TOPO_LIST, first, size, count, sign, 4, O.GRID_FRONTIER)
+        assert s == 0
+        return O.bound_check_counts()
+    finally:
+        O.set_bound_check(O.BOUND_OFF)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_plane_bound_skips_only_pairs_that_cannot_improve(oracle, suzanne, mode):
+    """The device propagation skips a (neighbour, triangle) pair when the
+    supporting-plane bound g + u.o already exceeds the neighbour's key
+    distance (plane_bound in sdfx_grid.hip, restated in FP32 by the oracle).
+    Every such pair of the frontier schedule is evaluated anyway and must not
+    lower its neighbour's key: with the direction words improvements write
+    (mode 1) and with arbitrary ones (mode 2, the bound holds for any unit
+    vector), on meshes, degenerate soups and slivers up to 10^4 from the
+    origin, both signs. With exact directions the bound skips at least a
+    third of the pairs it tests on the meshes."""
+    O = oracle
+    total = {"checked": 0, "skipped": 0, "wrong": 0}
+    for name, v, f, grid, sign in _bound_cases(suzanne):
+        c = _run_bound(O, v, f, grid, sign, mode)
+        assert c["wrong"] == 0, (name, c)
+        for k in total:
+            total[k] += c[k]
+        if mode == 1 and name in ("torus", "noise sphere", "suzanne"):
+            assert c["skipped"] >= c["checked"] // 3, (name, c)
+    print(f"mode {mode}: {total}")
+    assert total["skipped"] > 0
+
+
+def test_plane_bound_check_catches_a_wrong_margin(oracle, suzanne):
+    """A bound whose rounding margin is added instead of subtracted skips
+    pairs that would improve: the check sees it on the same cases."""
+    O = oracle
+    wrong = 0
+    O.set_fault(O.FAULT_BOUND_MARGIN_SIGN)
+    try:
+        for name, v, f, grid, sign in _bound_cases(suzanne):
+            wrong += _run_bound(O, v, f, grid, sign, O.BOUND_EXACT_DIRECTION)["wrong"]
+    finally:
+        O.set_fault(O.FAULT_NONE)
+    print(f"wrong margin sign: {wrong} skipped pairs that improve")
+    assert wrong > 0
