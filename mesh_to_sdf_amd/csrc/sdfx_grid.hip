@@ -76,6 +76,7 @@ struct GridReq {
     float far;         // a thin round with the band minimum beyond this makes every later round take all
     uint32_t release;  // after a round that took fewer cells, a round takes every flagged cell
     uint32_t flag;     // index of the request's DevFlags
+    uint32_t reach;    // bit k: axis k's cells are thin enough for two-step neighbours (grid_reach)
 };
 
 // Request lookup. A batch reads its table; a single request travels by value
@@ -260,7 +261,7 @@ __device__ __forceinline__ PlaneBound plane_bound(uint32_t w, float cx, float cy
     sc = fmaxf(sc, fmaxf(fmaxf(fabsf(t.ax), fabsf(t.ay)), fabsf(t.az)));
     sc = fmaxf(sc, fmaxf(fmaxf(fabsf(t.bx), fabsf(t.by)), fabsf(t.bz)));
     sc = fmaxf(sc, fmaxf(fmaxf(fabsf(t.cx), fabsf(t.cy)), fabsf(t.cz)));
-    sc = sc + max_size;
+    sc = sc + 2.0f * max_size;  // the farthest neighbour is at most two cells away
     const float e = sqrtf(fmaxf(ab2, fmaxf(ac2, bc2)));
     const float g = (dot3(ux, uy, uz, cx, cy, cz) - h) - (sc * (1.0f / 65536.0f) + e * (1.0f / 4096.0f));
     b.ux = ux;
@@ -309,6 +310,40 @@ constexpr float GRID_BAND_CELLS = 3.0f;
 // in the far field the front carries mostly final values and the band only
 // adds rounds.
 constexpr float GRID_FAR_CELLS = 8.0f;
+// A frontier item offers its triangle to the reference's 26 neighbours and,
+// along each axis whose cells are at least 1.5 times thinner than the largest,
+// to the two cells two steps away on that axis, so that a front moves about as
+// far along a thin axis per round as along the others (more candidates only
+// move values closer to the exact distance; the end state still satisfies the
+// reference's fixpoint over the 26).
+inline bool grid_reach(float max_size, float abs_size) { return max_size / abs_size >= 1.5f; }
+
+// Offset of neighbour k of an item (k < 26: the 26 around it, the centre
+// skipped; then two steps both ways along each axis whose bit is set in
+// reach); false when the request has fewer neighbours.
+__device__ __forceinline__ bool neighbour_offset(uint32_t k, uint32_t reach, int& ox, int& oy, int& oz) {
+    if (k < 26u) {
+        k += k >= 13u ? 1u : 0u;
+        ox = (int)(k / 9u) - 1;
+        oy = (int)((k / 3u) % 3u) - 1;
+        oz = (int)(k % 3u) - 1;
+        return true;
+    }
+    uint32_t e = k - 26u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!((reach >> a) & 1u)) continue;
+        if (e < 2u) {
+            const int d = e ? -2 : 2;
+            ox = a == 0 ? d : 0;
+            oy = a == 1 ? d : 0;
+            oz = a == 2 ? d : 0;
+            return true;
+        }
+        e -= 2u;
+    }
+    return false;
+}
 // A round after one that took fewer than max(GRID_RELEASE_MIN, cells /
 // GRID_RELEASE_DIV) cells takes every flagged cell: once the front is thin,
 // waiting for the band only adds rounds.
@@ -771,9 +806,11 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
 // it holds that many, so the distance evaluations run with every lane busy
 // instead of idling the lanes whose candidate was skipped.
 constexpr uint32_t PROP_QUEUE = 128;  // a fill adds at most 64 to at most 63 left over
-// Frontier items one propagation launch takes: 26 neighbour slots each plus
+// Frontier items one propagation launch takes: kmax neighbour slots each plus
 // one grid stride stay below 2^32.
-constexpr uint32_t PROP_CHUNK_ITEMS = (uint32_t)(((1ull << 32) - (1ull << 24)) / 26);
+__host__ __device__ constexpr uint32_t prop_chunk_items(uint32_t kmax) {
+    return (uint32_t)(((1ull << 32) - (1ull << 24)) / kmax);
+}
 constexpr uint32_t PROP_WAVES = 4;    // 256-thread blocks
 
 template <bool BATCH>
@@ -933,7 +970,7 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
                                                             uint32_t* __restrict__ taken_next,
                                                             uint32_t* __restrict__ total_out,
                                                             uint32_t* __restrict__ host_probe, uint32_t host_seq,
-                                                            uint32_t item_base,
+                                                            uint32_t item_base, uint32_t kmax,
                                                             DevFlags* flags,
                                                             unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
@@ -941,12 +978,12 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
     __shared__ uint32_t prefix[NCLASS + 1];
     __shared__ PropQueue<mode_batch<MODE>> Q;
     prop_prologue<MODE>(ctr_cur, prefix, nreq, min_cur, taken_next, total_out, host_probe, host_seq);
-    // this launch's items: [item_base, item_base + PROP_CHUNK_ITEMS) of the
-    // frontier (one launch covers the frontier of grids below 165M cells;
+    // this launch's items: [item_base, item_base + prop_chunk_items(kmax)) of
+    // the frontier (one launch covers the frontier of grids below 130M cells;
     // larger ones run several launches per round), so the 32-bit work index
-    // (26 neighbour slots per item) never overflows
+    // (kmax neighbour slots per item) never overflows
     const uint32_t nfront = prefix[NCLASS];
-    const uint32_t work = nfront > item_base ? min(nfront - item_base, PROP_CHUNK_ITEMS) * 26u : 0u;
+    const uint32_t work = nfront > item_base ? min(nfront - item_base, prop_chunk_items(kmax)) * kmax : 0u;
     // Each XCD (blocks b with b % XCDS == x) takes one contiguous eighth of the
     // items, which lie in spatial order, so keys and triangles of neighbouring
     // cells share that XCD's L2.
@@ -959,28 +996,29 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
     for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
         const uint32_t j = base + lane;
         bool keep = false;
-        uint32_t x = 0, y = 0, z = 0, tri = 0, q = 0;
+        uint32_t x = 0, y = 0, z = 0, tri = 0, q = 0, k = 0, ix = 0, iy = 0, iz = 0;
         uint64_t cur = 0;
         if (j < j1) {
-            const uint32_t f = item_base + j / 26u;
-            uint32_t k = j - f * 26u;
-            k += k >= 13u ? 1u : 0u;  // skip the centre (the cell itself never improves)
+            const uint32_t f = item_base + j / kmax;
+            k = j - (f - item_base) * kmax;
             uint32_t cls = 0;  // the last class whose prefix is <= f (a non-empty one)
 #pragma unroll
             for (uint32_t step = NCLASS / 2; step; step >>= 1)
                 if (prefix[cls + step] <= f) cls += step;
-            uint32_t ix, iy, iz;
             read_item<MODE>(items[(uint64_t)cls * region + (f - prefix[cls])], tri, q, ix, iy, iz);
-            x = ix + k / 9u - 1u;
-            y = iy + (k / 3u) % 3u - 1u;
-            z = iz + k % 3u - 1u;
         }
         // A neighbour already holding this triangle holds exactly the key the
         // triangle would give it, and keys only decrease.
         auto candidate = [&](const GridReq& R) {
-            if (j < j1 && x < R.g.n[0] && y < R.g.n[1] && z < R.g.n[2]) {  // -1 wraps to 2^32 - 1
-                cur = keys[R.cell_off + cell_index(R.g, x, y, z)];
-                keep = cur == KEY_EMPTY || key_tri(cur, SIGN) != tri;
+            int ox, oy, oz;
+            if (j < j1 && neighbour_offset(k, R.reach, ox, oy, oz)) {
+                x = ix + (uint32_t)ox;  // below 0 wraps past the axis length
+                y = iy + (uint32_t)oy;
+                z = iz + (uint32_t)oz;
+                if (x < R.g.n[0] && y < R.g.n[1] && z < R.g.n[2]) {
+                    cur = keys[R.cell_off + cell_index(R.g, x, y, z)];
+                    keep = cur == KEY_EMPTY || key_tri(cur, SIGN) != tri;
+                }
             }
         };
         if constexpr (mode_batch<MODE>) {
@@ -1049,7 +1087,7 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
                                                                   uint32_t* __restrict__ taken_next,
                                                                   uint32_t* __restrict__ total_out,
                                                                   uint32_t* __restrict__ host_probe, uint32_t host_seq,
-                                                                  uint32_t item_base, DevFlags* flags,
+                                                                  uint32_t item_base, uint32_t kmax, DevFlags* flags,
                                                                   unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
     __shared__ uint32_t prefix[NCLASS + 1];
@@ -1060,7 +1098,7 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
     __shared__ float4 s_bound[256];
     prop_prologue<MODE>(ctr_cur, prefix, nreq, min_cur, taken_next, total_out, host_probe, host_seq);
     const uint32_t nfront = prefix[NCLASS];
-    const uint32_t work = nfront > item_base ? min(nfront - item_base, PROP_CHUNK_ITEMS) : 0u;
+    const uint32_t work = nfront > item_base ? min(nfront - item_base, prop_chunk_items(kmax)) : 0u;
     // each XCD takes one contiguous eighth of the items (in spatial order)
     const uint32_t chunk = (work + XCDS - 1) / XCDS;
     const uint32_t j0 = (blockIdx.x % XCDS) * chunk, j1 = min(work, j0 + chunk);
@@ -1089,13 +1127,13 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
                 s_bound[threadIdx.x] = make_float4(pb.ux, pb.uy, pb.uz, pb.g);  // only this lane reads it
                 s_item[threadIdx.x] = make_uint4(ix, iy, iz, tri);
             }
-            for (uint32_t k = 0; k < 27u; ++k) {
-                if (k == 13u) continue;  // the cell itself never improves
+            for (uint32_t k = 0; k < kmax; ++k) {
                 bool keep = false;
                 uint32_t x = 0, y = 0, z = 0, t = 0;
                 uint64_t cur = 0;
-                const int ox = (int)(k / 9u) - 1, oy = (int)((k / 3u) % 3u) - 1, oz = (int)(k % 3u) - 1;
-                if (mine) {
+                int ox = 0, oy = 0, oz = 0;
+                const bool has = neighbour_offset(k, R.reach, ox, oy, oz);
+                if (mine && has) {
                     const uint4 it = s_item[threadIdx.x];
                     t = it.w;
                     x = it.x + (uint32_t)ox;  // -1 wraps past the axis length
@@ -1529,6 +1567,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     std::vector<GridReq> reqs(nreq);
     LeafRec* ltris = A.alloc<LeafRec>(nt);
     uint64_t ncells = 0, rows_total[3] = {0, 0, 0}, words_total[3] = {0, 0, 0}, max_extent = 0;
+    uint32_t kmax = 26;  // neighbours per item of the request with the most (neighbour_offset)
     for (uint32_t q = 0; q < nreq; ++q) {
         const GridBatchItem& J = jobs[q];
         GridReq& R = reqs[q];
@@ -1554,6 +1593,10 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         R.band = GRID_BAND_CELLS * max_size;
         R.far = GRID_FAR_CELLS * max_size;
         R.release = (uint32_t)std::max<uint64_t>(GRID_RELEASE_MIN, nc / GRID_RELEASE_DIV);
+        R.reach = 0;
+        for (int k = 0; k < 3; ++k)
+            if (grid_reach(max_size, std::fabs(R.g.size[k]))) R.reach |= 1u << k;
+        kmax = std::max<uint32_t>(kmax, 26u + 2u * (uint32_t)__builtin_popcount(R.reach));
         const uint64_t rows[3] = {J.gp.count[1] * J.gp.count[2], J.gp.count[0] * J.gp.count[2],
                                   J.gp.count[0] * J.gp.count[1]};
         for (int a = 0; a < 3; ++a) {
@@ -1676,8 +1719,9 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     // still complete.
     uint32_t last_taken = UINT32_MAX;
     // a round's frontier holds at most ncells items; each propagation launch
-    // takes PROP_CHUNK_ITEMS of them (one launch below 165M cells)
-    const uint32_t prop_chunks = (uint32_t)((ncells + PROP_CHUNK_ITEMS - 1) / PROP_CHUNK_ITEMS);
+    // takes prop_chunk_items(kmax) of them (one launch below 130M cells)
+    const uint32_t chunk_items = prop_chunk_items(kmax);
+    const uint32_t prop_chunks = (uint32_t)((ncells + chunk_items - 1) / chunk_items);
     auto enqueue_batch = [&](int slot) {
         const uint32_t seq = probe.arm(slot);
         const unsigned blocks = last_taken < 2048u ? std::min(prop_blocks, 256u) : prop_blocks;
@@ -1696,12 +1740,12 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
                 if (item_major)
                     SDFX_LAUNCH(k_frontier_propagate_items<MODE>, dim3(blocks), dim3(256), 0, s, items, region,
                                 ctr[cur], rv, nreq, keys, dirs, changed, tile_changed, minw[cur ^ 1], minw[cur],
-                                taken[cur ^ 1], totals + cur, probe_w, seq, k * PROP_CHUNK_ITEMS, d_flags,
+                                taken[cur ^ 1], totals + cur, probe_w, seq, k * chunk_items, kmax, d_flags,
                                 evals ? evals + seed_slots : nullptr);
                 else
                     SDFX_LAUNCH(k_frontier_propagate<MODE>, dim3(blocks), dim3(256), 0, s, items, region, ctr[cur],
                                 rv, nreq, keys, dirs, changed, tile_changed, minw[cur ^ 1], minw[cur], taken[cur ^ 1],
-                                totals + cur, probe_w, seq, k * PROP_CHUNK_ITEMS, d_flags,
+                                totals + cur, probe_w, seq, k * chunk_items, kmax, d_flags,
                                 evals ? evals + seed_slots : nullptr);
             }
             cur ^= 1;

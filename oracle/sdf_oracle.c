@@ -1093,7 +1093,7 @@ static void plane_bound_c(uint32_t w, v3 c, v3 A, v3 B, v3 C, float max_size, fl
     sc = fmaxf(sc, fmaxf(fmaxf(fabsf(A.x), fabsf(A.y)), fabsf(A.z)));
     sc = fmaxf(sc, fmaxf(fmaxf(fabsf(B.x), fabsf(B.y)), fabsf(B.z)));
     sc = fmaxf(sc, fmaxf(fmaxf(fabsf(C.x), fabsf(C.y)), fabsf(C.z)));
-    sc = sc + max_size;
+    sc = sc + 2.0f * max_size;  /* the farthest neighbour is at most two cells away */
     float e = sqrtf(fmaxf(ab2, fmaxf(ac2, bc2)));
     float margin = sc * (1.0f / 65536.0f) + e * (1.0f / 4096.0f);
     if (g_fault == ORC_FAULT_BOUND_MARGIN_SIGN) margin = -margin;
@@ -1127,6 +1127,8 @@ static void frontier_seed_triangle(const ogrid* g, const float* verts, const uin
             }
 }
 
+unsigned orc_grid_reach(float max_size, float abs_size) { return max_size / abs_size >= 1.5f ? 2u : 1u; }
+
 /* Returns the number of rounds, or -1 on a NaN distance (the reference panics). */
 static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint32_t* tris, uint64_t nt, int sign,
                                  float* dist_arr, uint32_t* tri_out, float* seed_out, int n_threads,
@@ -1146,6 +1148,8 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
     const float max_size = fmaxf(fmaxf(fabsf(g->size.x), fabsf(g->size.y)), fabsf(g->size.z));
     const float band = ORC_GRID_BAND_CELLS * max_size;
     const float far = ORC_GRID_FAR_CELLS * max_size;
+    const unsigned reach[3] = {orc_grid_reach(max_size, fabsf(g->size.x)), orc_grid_reach(max_size, fabsf(g->size.y)),
+                               orc_grid_reach(max_size, fabsf(g->size.z))};
     const uint64_t release = ncells / ORC_GRID_RELEASE_DIV > ORC_GRID_RELEASE_MIN ? ncells / ORC_GRID_RELEASE_DIV
                                                                                   : ORC_GRID_RELEASE_MIN;
     uint64_t prev_taken = UINT64_MAX; /* round 0 keeps the band */
@@ -1203,11 +1207,21 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
                 }
                 plane_bound_c(w, c, A, B, C, max_size, bound);
             }
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dz = -1; dz <= 1; ++dz) {
-                        if (!dx && !dy && !dz) continue; /* the cell itself can never improve */
-                        if (g_fault == ORC_FAULT_SKIP_DIRECTION && dx == 1 && !dy && !dz) continue;
+            for (int nb = 0; nb < 33; ++nb) {
+                        int dx, dy, dz;
+                        if (nb < 27) {
+                            dx = nb / 9 - 1;
+                            dy = (nb / 3) % 3 - 1;
+                            dz = nb % 3 - 1;
+                            if (!dx && !dy && !dz) continue; /* the cell itself can never improve */
+                        } else { /* 27 + 2a, 28 + 2a... : two steps along axis a, both ways, when its reach is 2 */
+                            int a = (nb - 27) / 2, sgn = ((nb - 27) & 1) ? -1 : 1;
+                            if (reach[a] < 2) continue;
+                            dx = a == 0 ? 2 * sgn : 0;
+                            dy = a == 1 ? 2 * sgn : 0;
+                            dz = a == 2 ? 2 * sgn : 0;
+                        }
+                        if (g_fault == ORC_FAULT_SKIP_DIRECTION && dx > 0 && !dy && !dz) continue;
                         int64_t x = cx + dx, y = cy + dy, z = cz + dz;
                         if (x < 0 || y < 0 || z < 0 || x >= (int64_t)g->n[0] || y >= (int64_t)g->n[1] ||
                             z >= (int64_t)g->n[2])
