@@ -180,6 +180,28 @@ __device__ __forceinline__ bool key_lower(uint64_t* slot, uint64_t key, uint64_t
     }
 }
 
+// key_lower inside one launch, where `seen` was read in the same launch:
+// whether this launch lowered the slot. A Raycast key's atomicMin is issued
+// without waiting for its old value. The answer is key < seen, which can
+// differ from the atomic's own outcome only when another lane lowered the
+// slot below key after seen was read, i.e. in this launch; that lane's
+// key < seen holds too, so "some lane of this launch answered true" is
+// exactly "this launch lowered the slot". Callers only use the answer to flag
+// the slot for the next round (the same set either way), to store a direction
+// word (any word is valid) and to fold the key into a running minimum over
+// the slots' new values (each lowered slot's final key is answered true by
+// the lane that wrote it, and every key answered true is at least its slot's
+// final key, so the minimum is the same). Normal keys keep the
+// compare-and-swap loop, which needs the old value.
+template <int SIGN>
+__device__ __forceinline__ bool key_lower_async(uint64_t* slot, uint64_t key, uint64_t seen) {
+    if (SIGN != SDFX_SIGN_RAYCAST) return key_lower<SIGN>(slot, key, seen);
+    if (!(key < seen)) return false;
+    __hip_atomic_fetch_min((unsigned long long*)slot, (unsigned long long)key, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);  // old value unused: the no-return form
+    return true;
+}
+
 __device__ __forceinline__ uint32_t key_tri(uint64_t k, int sign) {
     return sign == SDFX_SIGN_NORMAL ? (uint32_t)(k & 0x7fffffffu) : (uint32_t)k;
 }
@@ -580,7 +602,7 @@ __global__ void __launch_bounds__(256) k_seed_pairs(uint64_t nt, ReqView<mode_ba
                 if (key_valid(d)) {
                     uint64_t key = make_key<SIGN>(d, (uint32_t)(t - R.tri_off));
                     const uint64_t ci = R.cell_off + cell_index(R.g, x, y, z);
-                    if (key_lower<SIGN>(&keys[ci], key, keys[ci])) dirs[ci] = dir_pack(dx, dy, dz);
+                    if (key_lower_async<SIGN>(&keys[ci], key, keys[ci])) dirs[ci] = dir_pack(dx, dy, dz);
                 }
             });
         }
@@ -649,8 +671,11 @@ __global__ void __launch_bounds__(256) k_mark_seeded(const uint64_t* __restrict_
 // smallest distance among q's flagged cells (the minimum of q's words in
 // min_cur, complete when the round starts), or every flagged cell of q when
 // q's previous round took fewer than release_q cells (the sum of q's words in
-// taken_prev). Each taken cell becomes an item carrying the triangle of its
-// current key and its flag is cleared. Flagged cells beyond the band keep
+// taken_prev), and in every round of q once a thin round (one after a round
+// that took fewer than release_q cells) has happened beyond far_q or for the
+// second time (released[q]; thin_round[q] holds round_tag, this round's
+// number plus one, of q's first thin round). Each taken cell becomes an item
+// carrying the triangle of its current key and its flag is cleared. Flagged cells beyond the band keep
 // their flags; their smallest distance is folded into min_next and the number
 // of taken cells into taken_cur.
 //
@@ -672,7 +697,9 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
                                                           uint32_t* __restrict__ min_next,
                                                           const uint32_t* __restrict__ taken_prev,
                                                           uint32_t* __restrict__ taken_cur,
-                                                          uint32_t* __restrict__ released, int sign) {
+                                                          uint32_t* __restrict__ released,
+                                                          uint32_t* __restrict__ thin_round, uint32_t round_tag,
+                                                          int sign) {
     __shared__ uint32_t wave_tot[4], wave_min[4];
     __shared__ uint32_t block_base, band_min, prev_taken, far_released;
     if (blockIdx.x == 0 && threadIdx.x < NCLASS) ctr_next[threadIdx.x * WORD_STRIDE] = 0u;
@@ -738,7 +765,15 @@ __global__ void __launch_bounds__(256) k_frontier_compact(const uint64_t* __rest
         const bool thin = prev_taken < R.release;
         // Every block taking a tile of q this round decides the same; the flag
         // is read by the later rounds' compactions.
-        if (thin && mf > R.far && !far_released && threadIdx.x == 0) released[q] = 1u;
+        if (thin && !far_released && threadIdx.x == 0) {
+            if (mf > R.far) released[q] = 1u;
+            // The first thin round records itself (every block of it writes
+            // the same tag; a block may read the tag another block of this
+            // round wrote, which is no later round); a later one releases q.
+            const uint32_t t = thin_round[q];
+            if (t == 0u) thin_round[q] = round_tag;
+            else if (t != round_tag) released[q] = 1u;
+        }
         float limit = mf + R.band;
         // a NaN band (or no flagged cell), a thin previous front, or the far field: every flagged cell
         if (!(limit >= mf) || thin || far_released) limit = INFINITY;
@@ -873,7 +908,7 @@ __device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, 
     if (!key_valid(d)) return;
     const uint64_t key = make_key<SIGN>(d, tri);
     const uint64_t ci = R.cell_off + cell_index(R.g, x, y, z);
-    if (key_lower<SIGN>(&keys[ci], key, cur)) {
+    if (key_lower_async<SIGN>(&keys[ci], key, cur)) {
         dirs[ci] = dir_pack(dx, dy, dz);
         changed[ci] = 1;
         tile_changed[ci / COMPACT_TILE] = 1;
@@ -1642,7 +1677,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     runs.end[1] = runs.end[0] + 2 * req_words;     // band minima: none yet
     runs.end[2] = runs.end[1] + req_words;         // taken counts of even rounds: 0
     runs.end[3] = runs.end[2] + req_words;         // taken counts read by round 0: large, so the band applies
-    runs.end[4] = runs.end[3] + 3 + nreq;          // totals, far-field flags, the ray stage's list count: 0
+    runs.end[4] = runs.end[3] + 3 + 2 * nreq;      // totals, release flags, the ray stage's list count, first thin rounds: 0
     runs.end[5] = runs.end[4] + bit_words;         // ray-parity bits: 0
     runs.value[0] = 0u;
     runs.value[1] = UINT32_MAX;
@@ -1689,7 +1724,8 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
     uint32_t* ctr[2] = {words, words + cls_words};
     uint32_t* minw[2] = {words + 2 * cls_words, words + 2 * cls_words + req_words};
     uint32_t* taken[2] = {words + 2 * cls_words + 2 * req_words, words + 2 * cls_words + 3 * req_words};
-    uint32_t* released = totals + 2;  // per request: the far-field release happened
+    uint32_t* released = totals + 2;  // per request: every later round takes every flagged cell
+    uint32_t* thin_round = totals + 3 + nreq;  // per request: 1 + the round number of its first thin round, or 0
     const uint32_t region = (uint32_t)((ntiles + NCLASS - 1) / NCLASS * COMPACT_TILE);  // items per class
     uint8_t* changed = A.alloc<uint8_t>(ncells);  // whole tiles: 16-byte vector reads stay inside
     uint8_t* tile_changed = A.alloc<uint8_t>(ntiles);
@@ -1734,7 +1770,7 @@ static void grid_run(uint64_t nt, const std::vector<GridBatchItem>& jobs, DevFla
         for (int b = 0; b < batch; ++b, ++round) {
             SDFX_LAUNCH(k_frontier_compact<MODE>, dim3(compact_blocks), dim3(256), 0, s, keys, rv, tile_req, ntiles,
                         changed, tile_changed, items, region, ctr[cur], ctr[cur ^ 1], minw[cur], minw[cur ^ 1],
-                        taken[cur ^ 1], taken[cur], released, (int)SIGN);
+                        taken[cur ^ 1], taken[cur], released, thin_round, (uint32_t)(round + 1), (int)SIGN);
             for (uint32_t k = 0; k < prop_chunks; ++k) {
                 uint32_t* probe_w = b == batch - 1 && k + 1 == prop_chunks ? probe.probe(slot) : nullptr;
                 if (item_major)

@@ -1153,7 +1153,8 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
     const uint64_t release = ncells / ORC_GRID_RELEASE_DIV > ORC_GRID_RELEASE_MIN ? ncells / ORC_GRID_RELEASE_DIV
                                                                                   : ORC_GRID_RELEASE_MIN;
     uint64_t prev_taken = UINT64_MAX; /* round 0 keeps the band */
-    int released = 0;                 /* a thin round beyond `far` happened: every later round takes all */
+    int released = 0;                 /* every later round takes every flagged cell */
+    int thin_seen = 0;                /* a thin round has happened */
     int64_t rounds = 0;
     uint64_t n_evals = 0;
     for (;;) {
@@ -1165,7 +1166,10 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
         float mf;
         memcpy(&mf, &m, 4);
         float limit = mf + band;
-        if (prev_taken < release && mf > far) released = 1;
+        if (prev_taken < release) {
+            if (mf > far || thin_seen) released = 1; /* a thin round in the far field, or the second one */
+            thin_seen = 1;
+        }
         /* a NaN band takes every flagged cell */
         if (!(limit >= mf) || prev_taken < release || released) limit = INFINITY;
         uint64_t nf = 0;

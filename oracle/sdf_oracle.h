@@ -116,7 +116,12 @@ typedef struct {
  * only adds rounds. When such a thin round happens with m_r beyond
  * ORC_GRID_FAR_CELLS * max|cell_size| (FP32), every later round takes every
  * flagged cell: in the far field the front carries mostly final values, so the
- * band there only adds rounds. It is deterministic and is the schedule the
+ * band there only adds rounds. So does a second thin round wherever it
+ * happens: once a released round has been followed by a banded round that
+ * took almost nothing, the band only alternates such rounds with released
+ * ones (on the 128^3 bench grid, 52 rounds become 45, for 0.1% more
+ * evaluations and 96 fewer cells beyond 1e-5 x diagonal of the exact
+ * distance). It is deterministic and is the schedule the
  * device pipeline runs, so it pins the device kernels bit-exactly; its
  * distance from the reference's own schedules is measured separately.
  * schedule ORC_GRID_EXACT: not one of the reference's schedules either: the
