@@ -196,7 +196,12 @@ sdfx_status sdfx_query_sdf(const sdfx_mesh* mesh, const float* query_points, uin
  * SDFX_ERR_INVALID_ARGUMENT). Points and outputs are device pointers, work is
  * enqueued on `stream` (a hipStream_t, NULL for the null stream), and the call
  * returns when it is done. Calls on one device share that scratch, so the
- * library runs them one after another. */
+ * library runs them one after another. While `stream` is being captured
+ * (hipStreamBeginCapture), the call records its work into the graph and
+ * returns SDFX_OK at once: the device-side checks (a NaN distance, which the
+ * reference panics on) are then not reported, and every replay uses the same
+ * scratch, so replays that overlap with each other or with other calls on one
+ * GPU race. */
 sdfx_status sdfx_query_sdf_noalloc(const sdfx_mesh* mesh, const float* d_query_points, uint64_t query_count,
                                    float* d_out, uint64_t* out_count, void* stream);
 

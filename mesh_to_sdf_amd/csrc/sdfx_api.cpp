@@ -609,10 +609,27 @@ sdfx_status sdfx_query_sdf_noalloc(const sdfx_mesh* mesh, const float* d_query_p
         void* base = noalloc_scratch(mesh->device);
         Arena A(s, base, NOALLOC_SCRATCH_BYTES);
         DevFlags* flags = new_flags(A);
-        if (mesh->method == SDFX_ACCEL_NONE)
-            none_query(mesh->tris.get(), mesh->n_tris, d_query_points, query_count, mesh->sign, d_out, flags, A);
-        else
-            lbvh_query(mesh->bvh, d_query_points, query_count, mode_for(mesh->method, mesh->sign), d_out, flags, A);
+        // Under stream capture the work is only recorded: the device-side
+        // checks cannot be read back before a replay has run, so they go
+        // unreported (every kernel and fill here is capturable: no memset
+        // nodes, no host synchronisation).
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+        SDFX_HIP(hipStreamIsCapturing(s, &capture));
+        try {
+            if (mesh->method == SDFX_ACCEL_NONE)
+                none_query(mesh->tris.get(), mesh->n_tris, d_query_points, query_count, mesh->sign, d_out, flags, A);
+            else
+                lbvh_query(mesh->bvh, d_query_points, query_count, mode_for(mesh->method, mesh->sign), d_out, flags,
+                           A);
+        } catch (...) {
+            // work already enqueued on the scratch ends before the next call may reuse it
+            if (capture != hipStreamCaptureStatusActive) (void)hipStreamSynchronize(s);
+            throw;
+        }
+        if (capture == hipStreamCaptureStatusActive) {
+            if (out_count) *out_count = query_count;
+            return SDFX_OK;
+        }
         DevFlags f = read_flags(flags, s);
         check_internal(f);
         if (f.nan_distance) fail_nan();

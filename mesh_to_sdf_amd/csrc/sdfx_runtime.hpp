@@ -126,7 +126,12 @@ struct Arena {
     // host: the data goes through a pinned block the Arena keeps until it has
     // synchronised.
     void upload(void* d_dst, const void* h_src, size_t bytes);
+    // Waits for the stream before returning its blocks. An Arena over a fixed
+    // region owns nothing to return, so it does not wait: its caller decides
+    // when the region's work is done (and under stream capture no wait is
+    // allowed).
     ~Arena() {
+        if (region && blocks.empty() && staged.empty()) return;
         (void)hipStreamSynchronize(stream);
         for (void* p : blocks) dev_free(p);
         for (void* p : staged) pinned_free(p);
