@@ -30,7 +30,8 @@ TOPO_LIST, TOPO_STRIP = 0, 1
 SIGN_RAYCAST, SIGN_NORMAL = 0, 1
 ACCEL_NONE, ACCEL_BVH, ACCEL_RTREE, ACCEL_RTREE_BVH = 0, 1, 2, 3
 GRID_SEQUENTIAL, GRID_THREADED, GRID_FRONTIER, GRID_EXACT = 0, 1, 2, 3
-FAULT_NONE, FAULT_SKIP_DIRECTION, FAULT_DROP_DEFERRED, FAULT_PACKED_ORDER, FAULT_BOUND_MARGIN_SIGN = 0, 1, 2, 3, 4
+(FAULT_NONE, FAULT_SKIP_DIRECTION, FAULT_DROP_DEFERRED, FAULT_PACKED_ORDER, FAULT_BOUND_MARGIN_SIGN,
+ FAULT_NO_SECOND_RELEASE) = 0, 1, 2, 3, 4, 5
 BOUND_OFF, BOUND_EXACT_DIRECTION, BOUND_RANDOM_DIRECTION = 0, 1, 2
 NO_TRIANGLE = 0xFFFFFFFF
 

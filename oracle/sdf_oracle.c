@@ -1167,7 +1167,8 @@ static int64_t frontier_schedule(const ogrid* g, const float* verts, const uint3
         memcpy(&mf, &m, 4);
         float limit = mf + band;
         if (prev_taken < release) {
-            if (mf > far || thin_seen) released = 1; /* a thin round in the far field, or the second one */
+            /* a thin round in the far field, or the second one */
+            if (mf > far || (thin_seen && g_fault != ORC_FAULT_NO_SECOND_RELEASE)) released = 1;
             thin_seen = 1;
         }
         /* a NaN band takes every flagged cell */

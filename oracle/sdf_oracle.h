@@ -170,9 +170,12 @@ int orc_grid_fixpoint_check(const float* vertices, uint64_t n_vertices, const ui
  * the flagged cells beyond a round's band instead of keeping them,
  * ORC_FAULT_PACKED_ORDER orders Normal-mode values by the packed key alone
  * (|d|, then negative after positive) instead of compare_distances, which
- * prefers a positive value over a negative one it approximately equals. */
+ * prefers a positive value over a negative one it approximately equals.
+ * ORC_FAULT_NO_SECOND_RELEASE leaves out the release after a second thin
+ * round (a schedule change, not a fault: tests use it to show which grids the
+ * rule changes). */
 enum { ORC_FAULT_NONE = 0, ORC_FAULT_SKIP_DIRECTION = 1, ORC_FAULT_DROP_DEFERRED = 2, ORC_FAULT_PACKED_ORDER = 3,
-       ORC_FAULT_BOUND_MARGIN_SIGN = 4 };
+       ORC_FAULT_BOUND_MARGIN_SIGN = 4, ORC_FAULT_NO_SECOND_RELEASE = 5 };
 void orc_set_fault(int kind);
 
 /* Test-only check of the device propagation's supporting-plane skip

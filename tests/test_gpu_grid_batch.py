@@ -75,3 +75,33 @@ def test_failing_request_is_isolated(sdf):
     with pytest.raises(_lib.SdfxError) as e:
         sdf.generate_grid_sdf_batch(reqs, 0)
     assert e.value.status == _lib.ERR_INDEX_OUT_OF_RANGE and "request 2" in e.value.message
+
+
+@pytest.mark.parametrize("sign", [0, 1])
+def test_second_thin_round_release_matches_oracle(sdf, oracle, sign):
+    """Grids whose result the release after a request's second thin round
+    changes: alone and in one batch, bit-exact against the oracle's frontier
+    schedule, which differs on them from the same schedule without the rule."""
+    from test_grid_fixpoint import second_release_grids
+    grids = list(second_release_grids())
+    reqs = [(v, sdf.Topology.TriangleList(np.asarray(f, np.uint32).reshape(-1)), sdf.Grid(first, size, count))
+            for v, f, (first, size, count) in grids]
+    batch = sdf.generate_grid_sdf_batch(reqs, sign)
+    differs = 0
+    try:
+        for i, (v, f, (first, size, count)) in enumerate(grids):
+            f = np.asarray(f, np.uint32).reshape(-1)
+            s, ref, _ = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, first, size, count, sign, 8,
+                                                 oracle.GRID_FRONTIER)
+            assert s == 0
+            alone = sdf.generate_grid_sdf(v, reqs[i][1], reqs[i][2], sign)
+            assert np.array_equal(bits(alone), bits(ref)), f"grid {i} alone"
+            assert np.array_equal(bits(batch[i]), bits(ref)), f"grid {i} in the batch"
+            oracle.set_fault(oracle.FAULT_NO_SECOND_RELEASE)
+            s, other, _ = oracle.generate_grid_sdf(v, f, oracle.TOPO_LIST, first, size, count, sign, 8,
+                                                   oracle.GRID_FRONTIER)
+            oracle.set_fault(oracle.FAULT_NONE)
+            differs += not np.array_equal(bits(other), bits(ref))
+    finally:
+        oracle.set_fault(oracle.FAULT_NONE)
+    assert differs >= 2

@@ -111,6 +111,47 @@ def test_small_grids_every_schedule_reaches_the_fixpoint(oracle, sign):
             assert _fix(O, v, f, first, size, count, sign, out, tri) == CLEAN, (len(f), heaps, sched)
 
 
+def second_release_grids():
+    """Grids whose result the release after a second thin round changes (the
+    band minimum keeps falling below the far-field distance after a first
+    release, fronts meeting in a torus' hole): two tori and two requests of
+    the batched workload."""
+    v, f = S.torus(n_major=128, n_minor=64)
+    yield v, f, S.grid_for(v, (80, 80, 80))
+    v, f = S.torus(n_major=64, n_minor=32)
+    yield v, f, S.grid_for(v, (96, 96, 96))
+    c4 = S.config4(24)
+    for i in (13, 23):
+        mv, mf, first, size, count = c4[i]
+        yield mv, mf.reshape(-1), (first, size, count)
+
+
+@pytest.mark.parametrize("sign", [0, 1])
+def test_second_thin_round_release(oracle, sign):
+    """The release after a request's second thin round takes fewer rounds on
+    these grids and changes the frontier schedule's result on at least two of
+    them (the rule is exercised), and both schedules, with and without it,
+    reach the reference's fixpoint."""
+    O = oracle
+    changed = 0
+    try:
+        for v, f, (first, size, count) in second_release_grids():
+            runs = {}
+            for fault in (O.FAULT_NONE, O.FAULT_NO_SECOND_RELEASE):
+                O.set_fault(fault)
+                s, out, st, tri = O.generate_grid_sdf(v, f, O.TOPO_LIST, first, size, count, sign, 8,
+                                                      O.GRID_FRONTIER, want_tri=True)
+                O.set_fault(O.FAULT_NONE)
+                assert s == 0
+                assert _fix(O, v, f, first, size, count, sign, out, tri) == CLEAN, (count, fault)
+                runs[fault] = (out, st.rounds)
+            assert runs[O.FAULT_NONE][1] < runs[O.FAULT_NO_SECOND_RELEASE][1], count
+            changed += not np.array_equal(runs[O.FAULT_NONE][0], runs[O.FAULT_NO_SECOND_RELEASE][0])
+    finally:
+        O.set_fault(O.FAULT_NONE)
+    assert changed >= 2
+
+
 def test_fixpoint_check_catches_planted_schedule_faults(oracle, suzanne):
     """A frontier schedule that never offers a triangle to the +x neighbour, or
     that drops the cells beyond a round's band, leaves violations; so does
