@@ -1046,6 +1046,9 @@ struct PointReq {
     uint32_t max_visits;
     uint32_t flag;
     uint32_t budget;  // nearest-search visits before the per-wave pass (batched queries)
+    uint64_t p_off;   // first launch position of the request: a multiple of 64 (k_lbvh_query_batch)
+    uint32_t nq;      // queries of the request
+    uint32_t pad;
 };
 
 // Entries per wave in the work pool of k_lbvh_coop (node, box d^2 bits).
@@ -1397,43 +1400,54 @@ __global__ void __launch_bounds__(256) k_point_morton_batch(const PointReq* __re
     vals[g] = (uint32_t)g;
 }
 
+// The request holding launch position i of k_lbvh_query_batch: the last one
+// whose p_off is <= i. i is wave-uniform, so the search runs on scalar loads.
+__device__ __forceinline__ uint32_t point_req_at(const PointReq* __restrict__ reqs, uint32_t nreq, uint64_t i) {
+    uint32_t lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (wave_uniform_load(reqs, mid).p_off <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The queries of many requests in one launch. Launch positions run request by
+// request, each request's first position a multiple of 64 (p_off), so every
+// wave serves one request: its fields, the tree pointers among them, sit in
+// scalar registers and the traversal is the single-request one of
+// k_lbvh_query. Position p_off + j takes the request's j-th query in sorted
+// order (batch query q_off + j when unsorted).
 template <int MODE>
-// 7 waves per SIMD (at most 72 VGPRs): left to itself the compiler takes 77
-// and 6 waves, which measured slower on the served workload.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
-                                                          uint64_t total, const uint32_t* __restrict__ order,
+// 8 waves per SIMD (at most 64 VGPRs), as k_lbvh_query.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_lbvh_query_batch(const PointReq* __restrict__ reqs, uint32_t nreq,
+                                                          uint64_t positions, const uint32_t* __restrict__ order,
                                                           DevFlags* flags, DeferRec* __restrict__ deferred,
                                                           uint32_t* n_deferred) {
-    const uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
+    // the wave's first position (with a Morton order, each XCD walks one
+    // contiguous stretch of it)
+    const uint64_t w0 = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x +
+                        __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     bool dfr = false;
     DeferRec rec;
-    if (i < total) {  // no early return: the wave appends its deferred queries together
-        const uint64_t g = order ? order[i] : i;
-        const uint32_t r = point_req_of(reqs, nreq, g);
-        TravStats ts;
-        // The query's index within its request waits in LDS across the
-        // traversal (all registers are taken there; 1 KB per block).
-        __shared__ uint32_t local_index[256];
-        auto run = [&](const PointReq& R) {
-            const uint64_t k = g - R.q_off;  // below 2^32: a batch holds fewer than 2^32 queries
-            local_index[threadIdx.x] = (uint32_t)k;
+    if (w0 < positions) {  // no early return: the wave appends its deferred queries together
+        const uint32_t r = point_req_at(reqs, nreq, w0);
+        const PointReq R = wave_uniform_load(reqs, r);
+        const uint32_t j = (uint32_t)(w0 - R.p_off) + (threadIdx.x & 63u);
+        if (j < R.nq) {
+            const uint64_t s = R.q_off + j;
+            const uint32_t k = (uint32_t)((order ? order[s] : s) - R.q_off);  // a batch holds fewer than 2^32 queries
+            TravStats ts;
             float ans;
             dfr = query_point<MODE>(R.nodes, R.leaves, R.root, R.max_visits, R.q[3 * k], R.q[3 * k + 1],
                                     R.q[3 * k + 2], flags + R.flag, ts, R.budget, deferred != nullptr, rec, ans);
-            const uint32_t kk = local_index[threadIdx.x];
-            if (dfr) rec.q = kk;
-            else R.out[kk] = ans;
-        };
-        // queries lie request by request: a wave nearly always holds one, whose
-        // fields (tree pointers among them) then sit in scalar registers
-        const uint32_t r0 = __builtin_amdgcn_readfirstlane(r);
-        if (__ballot(r != r0) == 0) {
-            const PointReq R = wave_uniform_load(reqs, r0);
-            run(R);
-        } else {
-            run(reqs[r]);
+            if (dfr) {
+                rec.q = k;
+                rec.req = r;
+            } else {
+                R.out[k] = ans;
+            }
         }
-        rec.req = r;
     }
     if (deferred) {  // kernel argument: uniform
         const uint64_t m = __ballot(dfr);
@@ -1454,13 +1468,15 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
     for (size_t c0 = 0; c0 < items.size(); c0 += CHUNK) {
         const size_t c1 = std::min(items.size(), c0 + CHUNK);
         std::vector<PointReq> h;
-        uint64_t total = 0;
+        uint64_t total = 0, positions = 0;
         for (size_t k = c0; k < c1; ++k) {
             const PointBatchItem& it = items[k];
             if (!it.nq || !it.bvh || !it.bvh->n) continue;
+            if (it.nq >= (1ull << 32)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "a point batch holds fewer than 2^32 queries");
             h.push_back(PointReq{it.bvh->nodes.get(), it.bvh->leaves.get(), it.d_q, it.d_out, total, it.bvh->root,
-                                 visit_cap(*it.bvh), it.flag, visit_budget(*it.bvh)});
+                                 visit_cap(*it.bvh), it.flag, visit_budget(*it.bvh), positions, (uint32_t)it.nq, 0u});
             total += it.nq;
+            positions += (it.nq + 63) / 64 * 64;  // every request starts on a wave
         }
         if (h.empty()) continue;
         if (total >= (1ull << 32)) throw Error(SDFX_ERR_INVALID_ARGUMENT, "a point batch holds fewer than 2^32 queries");
@@ -1478,7 +1494,7 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
             radix_sort_pairs(keys, vals, total, 24 + rbits, A);
             order = vals;
         }
-        const unsigned blocks = order ? xcd_grid(blocks_for(total, 256)) : blocks_for(total, 256);
+        const unsigned blocks = order ? xcd_grid(blocks_for(positions, 256)) : blocks_for(positions, 256);
         // long nearest searches finish one per wave, as in lbvh_query
         const bool split = mode != QM_NORMAL_FOLD;
         DeferRec* recs = nullptr;
@@ -1490,20 +1506,20 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
         }
         switch (mode) {
             case QM_RAYCAST_VOTE:
-                SDFX_LAUNCH(k_lbvh_query_batch<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags, recs, words);
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_RAYCAST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
+                            positions, order, d_flags, recs, words);
                 break;
             case QM_NORMAL_FOLD:
-                SDFX_LAUNCH(k_lbvh_query_batch<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags, recs, words);
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_NORMAL_FOLD>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
+                            positions, order, d_flags, recs, words);
                 break;
             case QM_NEAREST_SIGNED:
                 SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_SIGNED>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
-                            total, order, d_flags, recs, words);
+                            positions, order, d_flags, recs, words);
                 break;
             case QM_NEAREST_VOTE:
-                SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq, total,
-                            order, d_flags, recs, words);
+                SDFX_LAUNCH(k_lbvh_query_batch<QM_NEAREST_VOTE>, dim3(blocks), dim3(256), 0, s, d_reqs, nreq,
+                            positions, order, d_flags, recs, words);
                 break;
             case QM_CLOSEST_VECTOR:
                 throw Error(SDFX_ERR_INVALID_ARGUMENT, "closest-point vectors are not an SDF query mode");
