@@ -1111,7 +1111,7 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
 // where 26 dependent steps per lane are hidden by many waves. Same arguments
 // and round bookkeeping as k_frontier_propagate.
 template <int MODE>
-__global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const FrontierItem* __restrict__ items,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_frontier_propagate_items(const FrontierItem* __restrict__ items,
                                                                   uint32_t region, const uint32_t* __restrict__ ctr_cur,
                                                                   ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
                                                                   uint64_t* keys, uint32_t* __restrict__ dirs,
@@ -1126,7 +1126,7 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
                                                                   unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
     __shared__ uint32_t prefix[NCLASS + 1];
-    __shared__ PropQueue<mode_batch<MODE>> Q;
+    __shared__ PropQueue<false> Q;  // pairs of one request at a time
     // each lane's item and bound, read back at every neighbour so that they
     // need not stay in registers across the queue's evaluations
     __shared__ uint4 s_item[256];
@@ -1141,6 +1141,19 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     PropAcc acc;
     uint32_t qn = 0;  // queued pairs (the same value in every lane of the wave)
+    // The queue holds pairs of one request, qreq (wave-uniform): a walk for
+    // another request first evaluates what is queued. Every evaluation thus
+    // reads its request from scalar registers.
+    uint32_t qreq = 0;
+    auto drain = [&](const GridReq& R, uint32_t q0) {  // evaluates the qn < 64 queued pairs
+        if (lane < qn) {
+            const uint64_t cv = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
+            prop_eval_req<MODE>(R, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][lane],
+                                Q.y[wave][lane], Q.z[wave][lane], Q.tri[wave][lane], q0, cv, acc);
+        }
+        qn = 0;
+        wave_lds_sync();
+    };
     for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
         const uint32_t j = base + lane;
         const bool live = j < j1;
@@ -1153,7 +1166,7 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
                 if (prefix[cls + step] <= f) cls += step;
             read_item<MODE>(items[(uint64_t)cls * region + (f - prefix[cls])], tri, q, ix, iy, iz);
         }
-        auto walk = [&](const GridReq& R, bool mine) {
+        auto walk = [&](const GridReq& R, uint32_t q0, bool mine) {
             if (mine) {
                 const float max_size = fmaxf(fmaxf(fabsf(R.g.size[0]), fabsf(R.g.size[1])), fabsf(R.g.size[2]));
                 const PlaneBound pb = plane_bound(dirs[R.cell_off + cell_index(R.g, ix, iy, iz)],
@@ -1199,7 +1212,6 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
                     Q.y[wave][pos] = y;
                     Q.z[wave][pos] = z;
                     Q.tri[wave][pos] = t;
-                    if constexpr (mode_batch<MODE>) Q.req[wave][pos] = q;
                     Q.cur_lo[wave][pos] = (uint32_t)cur;
                     Q.cur_hi[wave][pos] = (uint32_t)(cur >> 32);
                 }
@@ -1209,9 +1221,8 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
                     qn -= 64u;
                     const uint32_t e = qn + lane;
                     const uint64_t cv = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
-                    prop_eval<MODE>(reqs, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][e],
-                                    Q.y[wave][e], Q.z[wave][e], Q.tri[wave][e],
-                                    mode_batch<MODE> ? Q.req[wave][e] : 0u, cv, acc);
+                    prop_eval_req<MODE>(R, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][e],
+                                        Q.y[wave][e], Q.z[wave][e], Q.tri[wave][e], q0, cv, acc);
                     wave_lds_sync();
                 }
             }
@@ -1224,17 +1235,18 @@ __global__ void __launch_bounds__(256, 6) k_frontier_propagate_items(const Front
             while (pending) {
                 const uint32_t q0 = __builtin_amdgcn_readlane(q, __ffsll((unsigned long long)pending) - 1);
                 const bool mine = live && q == q0;
-                walk(wave_req(reqs.table, q0), mine);
+                if (q0 != qreq && qn) drain(wave_req(reqs.table, qreq), qreq);
+                qreq = q0;
+                walk(wave_req(reqs.table, q0), q0, mine);
                 pending &= ~__ballot(mine);
             }
         } else {
-            walk(reqs.one, live);
+            walk(reqs.one, 0u, live);
         }
     }
-    if (lane < qn) {
-        const uint64_t cv = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
-        prop_eval<MODE>(reqs, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][lane], Q.y[wave][lane],
-                        Q.z[wave][lane], Q.tri[wave][lane], mode_batch<MODE> ? Q.req[wave][lane] : 0u, cv, acc);
+    if (qn) {
+        if constexpr (mode_batch<MODE>) drain(wave_req(reqs.table, qreq), qreq);
+        else drain(reqs.one, 0u);
     }
     if (mode_batch<MODE>) fold_req_min<MODE>(min_next, acc.req, acc.imin, blockIdx.x);
     else block_min_u32(req_word<MODE>(min_next, 0, blockIdx.x), acc.imin);
