@@ -261,7 +261,10 @@ sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex
  *   "lbvh_visit_budget": LBVH node visits after which a nearest search stops
  *   in the per-query pass and finishes in a second pass that gives the query a
  *   whole wave, 16..2^31-1, or 0 (the default) for 15 x cbrt(triangles) per
- *   mesh. */
+ *   mesh.
+ *   "grid_item_major_min": grid rounds after one that took at least this many
+ *   cells propagate one lane per frontier cell rather than one lane per
+ *   (cell, neighbour) pair, >= 0 (0: every round; default 2^20). */
 int sdfx_set_tuning(const char* key, int64_t value);
 int64_t sdfx_get_tuning(const char* key);
 
