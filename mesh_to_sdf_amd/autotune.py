@@ -139,7 +139,7 @@ def tune_grid_blocks(L, sp, log, candidates=(1024, 2048, 4096, 6144, 8192)):
     return min(total, key=total.get), {str(c): t * 1e3 for c, t in total.items()}
 
 
-def tune_batch_grid_blocks(L, sp, log, candidates=(1024, 2048, 3072, 4096)):
+def tune_batch_grid_blocks(L, sp, log, candidates=(1024, 1536, 1792, 2048, 3072, 4096)):
     """grid_batch_prop_blocks, timed on the first 32 requests of the batched
     workload (one generate_grid_sdf_batch call each; the host copies are the
     same for every candidate)."""
