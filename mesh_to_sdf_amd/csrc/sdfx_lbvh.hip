@@ -1068,7 +1068,10 @@ constexpr uint32_t COOP_POOL = 128;
 // why the order, and visiting a subtree twice, do not matter). Persistent
 // waves take queries from *next until all *n_deferred are done.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_lbvh_coop(const PointReq* __restrict__ reqs, PointReq single,
+// 6 waves per SIMD (at most 80 VGPRs, 20-24 bytes of spills per lane): left to
+// itself the compiler takes 87-89 VGPRs and 5 waves, measured 1-2% slower on
+// the noise-sphere workload.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_lbvh_coop(const PointReq* __restrict__ reqs, PointReq single,
                                                    const DeferRec* __restrict__ recs, const uint32_t* n_deferred,
                                                    uint32_t* next, DevFlags* flags, unsigned long long* stats) {
     __shared__ uint32_t pool_node[4][COOP_POOL], pool_d2[4][COOP_POOL];
