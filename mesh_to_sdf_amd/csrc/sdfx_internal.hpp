@@ -239,16 +239,21 @@ __device__ __forceinline__ void atomic_min_if_lower(uint32_t* out, uint32_t m) {
     if (m < __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(out, m);
 }
 // MI355X dispatches the blocks of a launch to its 8 XCDs round-robin (block b
-// runs on XCD b % 8), and each XCD has its own L2. xcd_block() renumbers the
-// blocks so each XCD takes one contiguous eighth of the block range: work laid
-// out in spatial order then shares each L2 between neighbouring items. The
-// launch must have gridDim.x = 8 * ceil(n / 8) blocks; results >= n are idle.
+// runs on XCD b % 8), and each XCD has its own L2. xcd_chunk_block() renumbers
+// the blocks in stretches of `chunk` blocks dealt to the XCDs in turn (XCD x
+// takes stretches x, x + 8, ...): each stretch of work laid out in spatial
+// order shares one L2, and work that is heavier in one part of the range
+// spreads over every XCD. The launch must have xcd_chunk_grid(n, chunk) blocks
+// (a multiple of 8 * chunk); results >= n are idle.
 constexpr uint32_t XCDS = 8;
-__device__ __forceinline__ uint32_t xcd_block() {
-    const uint32_t per = gridDim.x / XCDS;
-    return (blockIdx.x % XCDS) * per + blockIdx.x / XCDS;
+__device__ __forceinline__ uint32_t xcd_chunk_block(uint32_t chunk) {
+    const uint32_t x = blockIdx.x % XCDS, i = blockIdx.x / XCDS;
+    return ((i / chunk) * XCDS + x) * chunk + i % chunk;
 }
-inline unsigned xcd_grid(uint64_t blocks) { return (unsigned)((blocks + XCDS - 1) / XCDS * XCDS); }
+inline unsigned xcd_chunk_grid(uint64_t blocks, uint32_t chunk) {
+    const uint64_t step = (uint64_t)XCDS * chunk;
+    return (unsigned)((blocks + step - 1) / step * step);
+}
 // Block-wide minimum of v folded into *out (nothing when every lane holds
 // UINT32_MAX). All lanes of the block must call it.
 __device__ __forceinline__ void block_min_u32(uint32_t* out, uint32_t v) {

@@ -461,6 +461,12 @@ void lbvh_build(const TriRec* d_tris, uint64_t n, Lbvh& bvh, Arena& A) {
 // was dropped without other storage, so no traversal touches scratch memory.
 // The traversal kernels run 256-thread blocks: 8 entries x 8 bytes x 256
 // lanes = 16 KB of LDS per block keeps 8 waves per SIMD resident.
+// Blocks per stretch of the XCD mapping of the Morton-ordered query kernels
+// (xcd_chunk_block). Stretches of 1 to 16 blocks measured alike; one
+// contiguous eighth of the queries per XCD left the XCDs unevenly loaded
+// (served burst 69.6 -> 60.0 ms, noise-sphere Bvh(Raycast) 24.2 -> 23.3 ms,
+// Rtree 20.9 -> 21.3 ms).
+constexpr uint32_t QUERY_XCD_CHUNK = 8;
 constexpr int STACK_RING = 8;
 constexpr int STACK_LANES = 256;
 
@@ -990,8 +996,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                                                     const uint32_t* __restrict__ order, float* __restrict__ out,
                                                     DevFlags* flags, unsigned long long* stats, uint32_t budget,
                                                     DeferRec* __restrict__ deferred, uint32_t* n_deferred) {
-    // with a Morton order, each XCD walks one contiguous stretch of it
-    uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
+    // with a Morton order, the XCDs take stretches of QUERY_XCD_CHUNK blocks in turn
+    uint64_t i = (uint64_t)(order ? xcd_chunk_block(QUERY_XCD_CHUNK) : blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = i < nq;  // no early return: the statistics below use block barriers
     TravStats ts;
     bool dfr = false;
@@ -1298,7 +1304,7 @@ void lbvh_query(const Lbvh& bvh, const float* d_q, uint64_t nq, QueryMode mode, 
     if (!nq || !bvh.n) return;
     hipStream_t s = A.stream;
     const uint32_t* order = query_order(bvh, d_q, nq, A);
-    const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
+    const unsigned blocks = order ? xcd_chunk_grid(blocks_for(nq, 256), QUERY_XCD_CHUNK) : blocks_for(nq, 256);
     // Nearest searches longer than the visit budget finish in a second pass
     // that gives each of them a whole wave: a few interior queries can need
     // tens of thousands of visits, and one lane walking them alone would set
@@ -1427,9 +1433,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                                                           uint64_t positions, const uint32_t* __restrict__ order,
                                                           DevFlags* flags, DeferRec* __restrict__ deferred,
                                                           uint32_t* n_deferred) {
-    // the wave's first position (with a Morton order, each XCD walks one
-    // contiguous stretch of it)
-    const uint64_t w0 = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x +
+    // the wave's first position (with a Morton order, the XCDs take
+    // stretches of QUERY_XCD_CHUNK blocks in turn)
+    const uint64_t w0 = (uint64_t)(order ? xcd_chunk_block(QUERY_XCD_CHUNK) : blockIdx.x) * blockDim.x +
                         __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     bool dfr = false;
     DeferRec rec;
@@ -1497,7 +1503,7 @@ void lbvh_query_batch(const std::vector<PointBatchItem>& items, QueryMode mode, 
             radix_sort_pairs(keys, vals, total, 24 + rbits, A);
             order = vals;
         }
-        const unsigned blocks = order ? xcd_grid(blocks_for(positions, 256)) : blocks_for(positions, 256);
+        const unsigned blocks = order ? xcd_chunk_grid(blocks_for(positions, 256), QUERY_XCD_CHUNK) : blocks_for(positions, 256);
         // long nearest searches finish one per wave, as in lbvh_query
         const bool split = mode != QM_NORMAL_FOLD;
         DeferRec* recs = nullptr;
@@ -1544,7 +1550,7 @@ __global__ void __launch_bounds__(256) k_closest_vectors(const BvhNode* __restri
                                                          uint8_t* __restrict__ inside, DevFlags* flags,
                                                          uint32_t budget, DeferRec* __restrict__ deferred,
                                                          uint32_t* n_deferred) {
-    uint64_t i = (uint64_t)(order ? xcd_block() : blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t i = (uint64_t)(order ? xcd_chunk_block(QUERY_XCD_CHUNK) : blockIdx.x) * blockDim.x + threadIdx.x;
     bool dfr = false;
     DeferRec rec;
     if (i < nq) {  // no early return: the wave appends its deferred queries together
@@ -1579,7 +1585,7 @@ void lbvh_closest_vectors(const Lbvh& bvh, const float* d_q, uint64_t nq, float*
     if (!nq || !bvh.n) return;
     hipStream_t s = A.stream;
     const uint32_t* order = query_order(bvh, d_q, nq, A);
-    const unsigned blocks = order ? xcd_grid(blocks_for(nq, 256)) : blocks_for(nq, 256);
+    const unsigned blocks = order ? xcd_chunk_grid(blocks_for(nq, 256), QUERY_XCD_CHUNK) : blocks_for(nq, 256);
     const uint32_t budget = visit_budget(bvh);
     const bool split = bvh.root >= 0 && budget < visit_cap(bvh);
     DeferRec* recs = nullptr;
