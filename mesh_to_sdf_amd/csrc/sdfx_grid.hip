@@ -1019,16 +1019,16 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
     // (kmax neighbour slots per item) never overflows
     const uint32_t nfront = prefix[NCLASS];
     const uint32_t work = nfront > item_base ? min(nfront - item_base, prop_chunk_items(kmax)) * kmax : 0u;
-    // Each XCD (blocks b with b % XCDS == x) takes one contiguous eighth of the
-    // items, which lie in spatial order, so keys and triangles of neighbouring
-    // cells share that XCD's L2.
-    const uint32_t chunk = (work + XCDS - 1) / XCDS;
-    const uint32_t j0 = (blockIdx.x % XCDS) * chunk, j1 = min(work, j0 + chunk);
-    const uint32_t stride = gridDim.x / XCDS * blockDim.x;
+    // A plain grid-stride loop: consecutive blocks, on different XCDs, take
+    // consecutive stretches of pairs. Giving each XCD one contiguous eighth
+    // of the work instead, for L2 locality, left the XCDs unevenly loaded
+    // (128^3 grid 1.715 -> 1.686 ms, batched grids 35.8 -> 34.4 ms without it).
+    const uint32_t j1 = work;
+    const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     PropAcc acc;
     uint32_t qn = 0;  // queued pairs (the same value in every lane of the wave)
-    for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
+    for (uint32_t base = blockIdx.x * blockDim.x + wave * 64u; base < j1; base += stride) {
         const uint32_t j = base + lane;
         bool keep = false;
         uint32_t x = 0, y = 0, z = 0, tri = 0, q = 0, k = 0, ix = 0, iy = 0, iz = 0;
@@ -1134,10 +1134,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
     prop_prologue<MODE>(ctr_cur, prefix, nreq, min_cur, taken_next, total_out, host_probe, host_seq);
     const uint32_t nfront = prefix[NCLASS];
     const uint32_t work = nfront > item_base ? min(nfront - item_base, prop_chunk_items(kmax)) : 0u;
-    // each XCD takes one contiguous eighth of the items (in spatial order)
-    const uint32_t chunk = (work + XCDS - 1) / XCDS;
-    const uint32_t j0 = (blockIdx.x % XCDS) * chunk, j1 = min(work, j0 + chunk);
-    const uint32_t stride = gridDim.x / XCDS * blockDim.x;
+    // a plain grid-stride loop over the items, as in k_frontier_propagate
+    const uint32_t j1 = work;
+    const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     PropAcc acc;
     uint32_t qn = 0;  // queued pairs (the same value in every lane of the wave)
@@ -1154,7 +1153,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         qn = 0;
         wave_lds_sync();
     };
-    for (uint32_t base = j0 + blockIdx.x / XCDS * blockDim.x + wave * 64u; base < j1; base += stride) {
+    for (uint32_t base = blockIdx.x * blockDim.x + wave * 64u; base < j1; base += stride) {
         const uint32_t j = base + lane;
         const bool live = j < j1;
         uint32_t tri = 0, q = 0, ix = 0, iy = 0, iz = 0;
