@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--burst-only", action="store_true",
                     help="serve: time the burst steps only (no Poisson sweep or plain run), for profiler passes "
                          "that must see only the timed region's launches")
+    ap.add_argument("--serve-workers", type=int, default=2,
+                    help="serve: worker threads of the server (each with its own stream)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the reference for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -200,7 +202,7 @@ def main():
         config = {"workload": WORKLOADS["mesh_check"], "triangles": int(len(f)), "vertices": int(len(v)),
                   "unit_of_work": "triangles checked (host arrays in, verdict out)"}
     elif args.workload == "serve":
-        serve = ServeWorkload(torch, dev, L, sp)
+        serve = ServeWorkload(torch, dev, L, sp, workers=args.serve_workers)
         step = serve.burst
         samples, mesh_bytes = serve.samples, serve.mesh_bytes
         v, f = serve.meshes[0][0], serve.meshes[0][1]
