@@ -257,7 +257,7 @@ sdfx_status sdfx_check_mesh_triangle_list(const float* vertices, uint64_t vertex
  *   "grid_prop_blocks": blocks per grid propagation launch, 8..65536 (rounded
  *   down to a multiple of 8; default 4096).
  *   "grid_batch_prop_blocks": the same for batched grid calls (several
- *   requests per launch; default 1792).
+ *   requests per launch; default 1536).
  *   "lbvh_visit_budget": LBVH node visits after which a nearest search stops
  *   in the per-query pass and finishes in a second pass that gives the query a
  *   whole wave, 16..2^31-1, or 0 (the default) for 15 x cbrt(triangles) per

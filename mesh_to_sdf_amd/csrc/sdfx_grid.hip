@@ -855,6 +855,13 @@ struct PropQueue {
     uint32_t cur_lo[PROP_WAVES][PROP_QUEUE], cur_hi[PROP_WAVES][PROP_QUEUE];
 };
 
+// The queue of k_frontier_propagate_items: a pair is its item's lane in the
+// block (the item, its triangle and bound sit in LDS) and its neighbour slot,
+// src << 8 | k, with the neighbour's key read by the candidate stage.
+struct PropQueueRef {
+    uint32_t sk[PROP_WAVES][PROP_QUEUE], cur_lo[PROP_WAVES][PROP_QUEUE], cur_hi[PROP_WAVES][PROP_QUEUE];
+};
+
 // LDS traffic between lanes of one wave: keep the compiler from moving the
 // accesses across this point (the wave's LDS operations execute in order).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -872,13 +879,12 @@ struct PropAcc {
 // request q to its mesh-local triangle tri, kept if its key is below the
 // neighbour's current key.
 template <int MODE>
-__device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, uint32_t* __restrict__ dirs,
-                                              uint8_t* __restrict__ changed,
+__device__ __forceinline__ void prop_eval_tri(const GridReq& R, const TriRec& tr, uint64_t* keys,
+                                              uint32_t* __restrict__ dirs, uint8_t* __restrict__ changed,
                                               uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
                                               DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
                                               uint32_t q, uint64_t cur, PropAcc& acc) {
     constexpr int SIGN = mode_sign<MODE>;
-    const TriRec tr = expand_leaf(R.ltris[tri]);  // three 16-byte loads instead of six, the same values
     const float px = cell_coord(R.g, 0, x), py = cell_coord(R.g, 1, y), pz = cell_coord(R.g, 2, z);
     float dx, dy, dz, cx, cy, cz;
     closest_diff_flat(px, py, pz, tr, dx, dy, dz, cx, cy, cz);
@@ -919,6 +925,17 @@ __device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, 
         if (mode_batch<MODE>) acc.req = q;
         acc.imin = min(acc.imin, (uint32_t)(key >> 32));
     }
+}
+
+// prop_eval_tri with the triangle read from the request's 48-byte copies.
+template <int MODE>
+__device__ __forceinline__ void prop_eval_req(const GridReq& R, uint64_t* keys, uint32_t* __restrict__ dirs,
+                                              uint8_t* __restrict__ changed,
+                                              uint8_t* __restrict__ tile_changed, uint32_t* __restrict__ min_next,
+                                              DevFlags* flags, uint32_t x, uint32_t y, uint32_t z, uint32_t tri,
+                                              uint32_t q, uint64_t cur, PropAcc& acc) {
+    const TriRec tr = expand_leaf(R.ltris[tri]);  // three 16-byte loads instead of six, the same values
+    prop_eval_tri<MODE>(R, tr, keys, dirs, changed, tile_changed, min_next, flags, x, y, z, tri, q, cur, acc);
 }
 
 // prop_eval_req for lane's request q. In a batch the wave's lanes nearly always
@@ -1111,7 +1128,7 @@ __global__ void __launch_bounds__(256, 8) k_frontier_propagate(const FrontierIte
 // where 26 dependent steps per lane are hidden by many waves. Same arguments
 // and round bookkeeping as k_frontier_propagate.
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_frontier_propagate_items(const FrontierItem* __restrict__ items,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_frontier_propagate_items(const FrontierItem* __restrict__ items,
                                                                   uint32_t region, const uint32_t* __restrict__ ctr_cur,
                                                                   ReqView<mode_batch<MODE>> reqs, uint32_t nreq,
                                                                   uint64_t* keys, uint32_t* __restrict__ dirs,
@@ -1126,11 +1143,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
                                                                   unsigned long long* evals) {
     constexpr int SIGN = mode_sign<MODE>;
     __shared__ uint32_t prefix[NCLASS + 1];
-    __shared__ PropQueue<false> Q;  // pairs of one request at a time
-    // each lane's item and bound, read back at every neighbour so that they
-    // need not stay in registers across the queue's evaluations
+    __shared__ PropQueueRef Q;  // pairs of one request at a time
+    // each lane's item, bound and triangle, read back at every neighbour so
+    // that they need not stay in registers across the queue's evaluations;
+    // the queued pairs take their cell and triangle from here
     __shared__ uint4 s_item[256];
     __shared__ float4 s_bound[256];
+    __shared__ float s_tv[10][256];  // triangle a, b, c and the degeneracy code (as bits) per lane
     prop_prologue<MODE>(ctr_cur, prefix, nreq, min_cur, taken_next, total_out, host_probe, host_seq);
     const uint32_t nfront = prefix[NCLASS];
     const uint32_t work = nfront > item_base ? min(nfront - item_base, prop_chunk_items(kmax)) : 0u;
@@ -1144,12 +1163,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
     // another request first evaluates what is queued. Every evaluation thus
     // reads its request from scalar registers.
     uint32_t qreq = 0;
+    // evaluates queued pair e (its item's cell, triangle and the neighbour's key from LDS)
+    auto eval_pair = [&](const GridReq& R, uint32_t q0, uint32_t e) {
+        const uint32_t sk = Q.sk[wave][e], src = sk >> 8;
+        const uint64_t cv = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
+        int ox = 0, oy = 0, oz = 0;
+        neighbour_offset(sk & 0xFFu, R.reach, ox, oy, oz);
+        const uint4 it = s_item[src];
+        // the same values expand_leaf gives (the topology index is not used by the grid)
+        TriRec tr;
+        make_trirec(s_tv[0][src], s_tv[1][src], s_tv[2][src], s_tv[3][src], s_tv[4][src], s_tv[5][src],
+                    s_tv[6][src], s_tv[7][src], s_tv[8][src], 0u, tr);
+        tr.degen = fbits(s_tv[9][src]);
+        prop_eval_tri<MODE>(R, tr, keys, dirs, changed, tile_changed, min_next, flags, it.x + (uint32_t)ox,
+                            it.y + (uint32_t)oy, it.z + (uint32_t)oz, it.w, q0, cv, acc);
+    };
     auto drain = [&](const GridReq& R, uint32_t q0) {  // evaluates the qn < 64 queued pairs
-        if (lane < qn) {
-            const uint64_t cv = ((uint64_t)Q.cur_hi[wave][lane] << 32) | Q.cur_lo[wave][lane];
-            prop_eval_req<MODE>(R, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][lane],
-                                Q.y[wave][lane], Q.z[wave][lane], Q.tri[wave][lane], q0, cv, acc);
-        }
+        if (lane < qn) eval_pair(R, q0, lane);
         qn = 0;
         wave_lds_sync();
     };
@@ -1168,9 +1198,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         auto walk = [&](const GridReq& R, uint32_t q0, bool mine) {
             if (mine) {
                 const float max_size = fmaxf(fmaxf(fabsf(R.g.size[0]), fabsf(R.g.size[1])), fabsf(R.g.size[2]));
+                const LeafRec lf = R.ltris[tri];
+                s_tv[0][threadIdx.x] = lf.ax;
+                s_tv[1][threadIdx.x] = lf.ay;
+                s_tv[2][threadIdx.x] = lf.az;
+                s_tv[3][threadIdx.x] = lf.bx;
+                s_tv[4][threadIdx.x] = lf.by;
+                s_tv[5][threadIdx.x] = lf.bz;
+                s_tv[6][threadIdx.x] = lf.cx;
+                s_tv[7][threadIdx.x] = lf.cy;
+                s_tv[8][threadIdx.x] = lf.cz;
+                s_tv[9][threadIdx.x] = bitsf(lf.degen);
                 const PlaneBound pb = plane_bound(dirs[R.cell_off + cell_index(R.g, ix, iy, iz)],
                                                   cell_coord(R.g, 0, ix), cell_coord(R.g, 1, iy),
-                                                  cell_coord(R.g, 2, iz), expand_leaf(R.ltris[tri]), max_size);
+                                                  cell_coord(R.g, 2, iz), expand_leaf(lf), max_size);
                 s_bound[threadIdx.x] = make_float4(pb.ux, pb.uy, pb.uz, pb.g);  // only this lane reads it
                 s_item[threadIdx.x] = make_uint4(ix, iy, iz, tri);
             }
@@ -1207,10 +1248,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
                 const uint64_t m = __ballot(keep);
                 if (keep) {
                     const uint32_t pos = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    Q.x[wave][pos] = x;
-                    Q.y[wave][pos] = y;
-                    Q.z[wave][pos] = z;
-                    Q.tri[wave][pos] = t;
+                    Q.sk[wave][pos] = threadIdx.x << 8 | k;
                     Q.cur_lo[wave][pos] = (uint32_t)cur;
                     Q.cur_hi[wave][pos] = (uint32_t)(cur >> 32);
                 }
@@ -1218,10 +1256,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
                 wave_lds_sync();
                 if (qn >= 64u) {  // evaluate the newest 64 queued pairs
                     qn -= 64u;
-                    const uint32_t e = qn + lane;
-                    const uint64_t cv = ((uint64_t)Q.cur_hi[wave][e] << 32) | Q.cur_lo[wave][e];
-                    prop_eval_req<MODE>(R, keys, dirs, changed, tile_changed, min_next, flags, Q.x[wave][e],
-                                        Q.y[wave][e], Q.z[wave][e], Q.tri[wave][e], q0, cv, acc);
+                    eval_pair(R, q0, qn + lane);
                     wave_lds_sync();
                 }
             }
@@ -1242,10 +1277,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         } else {
             walk(reqs.one, 0u, live);
         }
-    }
-    if (qn) {
-        if constexpr (mode_batch<MODE>) drain(wave_req(reqs.table, qreq), qreq);
-        else drain(reqs.one, 0u);
+        // the next items replace these in LDS: evaluate what is queued
+        if (qn) {
+            if constexpr (mode_batch<MODE>) drain(wave_req(reqs.table, qreq), qreq);
+            else drain(reqs.one, 0u);
+        }
     }
     if (mode_batch<MODE>) fold_req_min<MODE>(min_next, acc.req, acc.imin, blockIdx.x);
     else block_min_u32(req_word<MODE>(min_next, 0, blockIdx.x), acc.imin);

@@ -207,9 +207,9 @@ namespace sdfx {
 Tuning& tuning() {
     // 2^26 pairs: about 10k queries against 6k triangles, where the exhaustive
     // kernel finishes well before an LBVH build and traversal would.
-    // 1,792 batched propagation blocks: one resident generation of the
-    // item-major kernel (7 blocks per CU at 7 waves per SIMD)
-    static Tuning t{int64_t(1) << 26, 4096, 0, 1792, 1 << 20};
+    // 1,536 batched propagation blocks: one resident generation of the
+    // item-major kernel (6 blocks per CU, bound by its 24.8 KB of LDS)
+    static Tuning t{int64_t(1) << 26, 4096, 0, 1536, 1 << 20};
     return t;
 }
 
