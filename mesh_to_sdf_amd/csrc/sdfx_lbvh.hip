@@ -881,15 +881,15 @@ __device__ __forceinline__ void ray_leaf(const LeafRec* __restrict__ leaves, int
     if ((axes & 4u) && ray_hit<2>(px, py, pz, t, nullptr)) cnt[2]++;
 }
 
-// Policy of dfs_exact for the 3-ray vote: the three rays share one traversal
-// and each path carries its live axes (a triangle must be counted exactly
-// once per ray, which the exact traversal guarantees).
+// Policy of dfs_exact for the 3-ray vote: the rays of root_pay's axes share
+// one traversal and each path carries its live axes (a triangle must be
+// counted exactly once per ray, which the exact traversal guarantees).
 struct VotePolicy {
     const LeafRec* __restrict__ leaves;
     float px, py, pz;
     TravStats& ts;
     uint32_t cnt[3] = {0u, 0u, 0u};
-    static constexpr uint32_t root_pay = 7u;
+    uint32_t root_pay = 7u;
     __device__ void count_node() { ts.vote_nodes++; }
     __device__ bool child(const float* lo, const float* hi, uint32_t axes, uint32_t& cp) {
         cp = axes & ray_axes(lo, hi, px, py, pz);
@@ -899,7 +899,11 @@ struct VotePolicy {
     __device__ void leaf(int32_t pos, uint32_t axes) { ray_leaf(leaves, pos, axes, px, py, pz, cnt, ts); }
 };
 
-// The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198.
+// The 3-ray vote of generic/bvh.rs:141-176 / generic/rtree_bvh.rs:163-198:
+// inside when at least two of the +x, +y, +z rays cross the surface an odd
+// number of times. The +x and +y rays go first; when their parities agree
+// they decide the vote whatever the +z ray counts, so the +z ray is traced
+// only where they disagree.
 __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const LeafRec* __restrict__ leaves, int32_t root,
                                 float px, float py, float pz, uint32_t max_visits, bool& overflow, TravStats& ts) {
     VotePolicy pol{leaves, px, py, pz, ts};
@@ -907,7 +911,13 @@ __device__ bool bvh_inside_vote(const BvhNode* __restrict__ nodes, const LeafRec
         pol.leaf(~root, 7u);
     } else {
         uint32_t visits = 0;
+        pol.root_pay = 3u;
         dfs_exact(nodes, root, -1, pol, visits, max_visits, max_visits, overflow, ts);
+        if ((pol.cnt[0] ^ pol.cnt[1]) & 1u) {
+            pol.root_pay = 4u;
+            visits = 0;
+            dfs_exact(nodes, root, -1, pol, visits, max_visits, max_visits, overflow, ts);
+        }
     }
     const int insides = (int)(pol.cnt[0] & 1u) + (int)(pol.cnt[1] & 1u) + (int)(pol.cnt[2] & 1u);
     return insides > 1;
