@@ -5,8 +5,8 @@
  * Every entry point below stands in for one function of the reference crate's
  * public Rust API; the comment above each cites the interface it replaces
  * (paths relative to mesh_to_sdf/src/ in ZiplineTeam/mesh_to_sdf). The
- * signatures use plain pointers and sizes only, so a Rust `-sys` crate (see
- * INTEGRATION.md), ctypes or cgo can bind them without HIP headers.
+ * signatures use plain pointers and sizes only, so a Rust `-sys` crate,
+ * ctypes or cgo can bind them without HIP headers.
  *
  * Data layout: vertices and query points are packed f32 triples (x, y, z),
  * i.e. `[[f32; 3]]`; indices are u32. Grid output is f32 per cell with z the
